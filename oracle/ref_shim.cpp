@@ -1,0 +1,209 @@
+// ref_shim.cpp -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+//
+// extern "C" entry points over the reference's own CPU classes, compiled
+// together with the reference's unmodified C++ sources into
+// oracle/_ref/libuhann_ref.so (recipe: oracle/Makefile, target `ref`). It lets
+// the tests pin the CPU restatement in uhann_oracle.cpp against the real
+// reference arithmetic, and lets the benchmark time the reference CPU path.
+// This file contains no reference code; it only calls the reference's public
+// API and two of its build-time helpers (detail::build_*_hierarchical_projection*).
+
+#include "ultrahigh_ann/coordinate_sampling/coordinate_sampling.hpp"
+#include "ultrahigh_ann/core/dense_l2_scan_index.hpp"
+#include "ultrahigh_ann/core/dense_matrix.hpp"
+#include "ultrahigh_ann/exact/l1/exact_l1_index.hpp"
+#include "ultrahigh_ann/exact/l2/exact_l2_index.hpp"
+#include "ultrahigh_ann/hnsvw25/l1/flat/l1_ann_index.hpp"
+#include "ultrahigh_ann/hnsvw25/l1/hierarchical/hierarchical_l1_ann_index.hpp"
+#include "ultrahigh_ann/hnsvw25/l1/importance_sampling.hpp"
+#include "ultrahigh_ann/hnsvw25/l2/flat/l2_ann_index.hpp"
+#include "ultrahigh_ann/hnsvw25/l2/hierarchical/hierarchical_l2_ann_index.hpp"
+#include "ultrahigh_ann/hnsvw25/l2/importance_sampling.hpp"
+
+#include "hnsvw25/l1/hierarchical/l1_hierarchical_projection.hpp"
+#include "hnsvw25/l2/hierarchical/l2_hierarchical_projection.hpp"
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <random>
+#include <span>
+#include <vector>
+
+namespace uh = ultrahigh_ann;
+
+namespace {
+
+uh::DenseMatrix make_matrix(const float* values, uint64_t rows, uint64_t cols)
+{
+    std::vector<float> data(values, values + rows * cols);
+    return uh::DenseMatrix(std::move(data), rows, cols);
+}
+
+template <class F> int guarded(F&& body)
+{
+    try {
+        body();
+        return 0;
+    } catch (const std::exception&) {
+        return 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ExactL2Index::query_batch with CpuDenseL2QueryStrategy: 0 sequential,
+// 2 parallel_queries (dense_l2_scan_index.hpp enum order checked in tests).
+int ref_exact_l2(const float* refs, uint64_t r, uint64_t d, const float* queries,
+                 uint64_t b, int parallel, uint64_t* out)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        const uh::ExactL2Index index(m);
+        std::vector<std::size_t> res(b);
+        index.query_batch(std::span<const float>(queries, b * d), b, res,
+                          parallel ? uh::CpuDenseL2QueryStrategy::parallel_queries
+                                   : uh::CpuDenseL2QueryStrategy::sequential);
+        for (uint64_t i = 0; i < b; ++i) out[i] = res[i];
+    });
+}
+
+int ref_exact_l1(const float* refs, uint64_t r, uint64_t d, const float* queries,
+                 uint64_t b, uint64_t* out)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        const uh::ExactL1Index index(m);
+        for (uint64_t i = 0; i < b; ++i) {
+            out[i] = index.query(std::span<const float>(queries + i * d, d));
+        }
+    });
+}
+
+int64_t ref_build_coordinate_sample(const double* probs, uint64_t d, uint64_t t,
+                                    uint64_t seed, uint64_t* cols, uint64_t* mult,
+                                    double* invp)
+{
+    try {
+        std::mt19937_64 eng(seed);
+        const auto s = uh::build_coordinate_sample(std::span<const double>(probs, d),
+                                                   t, eng);
+        for (std::size_t i = 0; i < s.columns.size(); ++i) {
+            cols[i] = s.columns[i].source_column;
+            mult[i] = s.columns[i].multiplicity;
+            invp[i] = s.columns[i].inverse_probability;
+        }
+        return static_cast<int64_t>(s.columns.size());
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_flat(int metric_l1, const float* refs, uint64_t r, uint64_t d,
+             const double* probs, uint64_t t, uint64_t seed, const float* queries,
+             uint64_t b, uint64_t* out)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        std::mt19937_64 eng(seed);
+        const std::span<const double> p(probs, d);
+        if (metric_l1) {
+            const uh::FlatL1AnnIndex index(m, p, t, eng);
+            for (uint64_t i = 0; i < b; ++i) {
+                out[i] = index.query(std::span<const float>(queries + i * d, d));
+            }
+        } else {
+            const uh::FlatL2AnnIndex index(m, p, t, eng);
+            std::vector<std::size_t> res(b);
+            index.query_batch(std::span<const float>(queries, b * d), b, res);
+            for (uint64_t i = 0; i < b; ++i) out[i] = res[i];
+        }
+    });
+}
+
+int ref_hierarchical(int metric_l1, const float* refs, uint64_t r, uint64_t d,
+                     const double* probs, uint64_t t, uint64_t p, uint64_t seed,
+                     const float* queries, uint64_t b, uint64_t* out)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        std::mt19937_64 eng(seed);
+        const std::span<const double> pr(probs, d);
+        if (metric_l1) {
+            const uh::HierarchicalL1AnnIndex index(m, pr, t, p, eng);
+            for (uint64_t i = 0; i < b; ++i) {
+                out[i] = index.query(std::span<const float>(queries + i * d, d));
+            }
+        } else {
+            const uh::HierarchicalL2AnnIndex index(m, pr, t, p, eng);
+            for (uint64_t i = 0; i < b; ++i) {
+                out[i] = index.query(std::span<const float>(queries + i * d, d));
+            }
+        }
+    });
+}
+
+// The FP32 device-side construction of the CUDA hierarchies
+// (build_l{1,2}_hierarchical_projection_fp32): sample, projection matrix and
+// projected references, with the engine seeded like the index constructors.
+// Buffers: cols/mult/invp of capacity d, proj32 p x k, refs32 r x p.
+int64_t ref_hierarchical_fp32_build(int metric_l1, const float* refs, uint64_t r,
+                                    uint64_t d, const double* probs, uint64_t t,
+                                    uint64_t p, uint64_t seed, uint64_t* cols,
+                                    uint64_t* mult, double* invp, float* proj32,
+                                    float* refs32)
+{
+    try {
+        const auto m = make_matrix(refs, r, d);
+        std::mt19937_64 eng(seed);
+        auto sample = uh::build_coordinate_sample(std::span<const double>(probs, d),
+                                                  t, eng);
+        auto copy_out = [&](const auto& built) {
+            const auto& cs = built.importance_sample.columns;
+            for (std::size_t i = 0; i < cs.size(); ++i) {
+                cols[i] = cs[i].source_column;
+                mult[i] = cs[i].multiplicity;
+                invp[i] = cs[i].inverse_probability;
+            }
+            return static_cast<int64_t>(cs.size());
+        };
+        if (metric_l1) {
+            auto built = uh::detail::build_l1_hierarchical_projection_fp32(
+                m, std::move(sample), p, eng);
+            const int64_t k = copy_out(built);
+            std::memcpy(proj32, built.cauchy_matrix.values().data(),
+                        built.cauchy_matrix.values().size_bytes());
+            std::memcpy(refs32, built.projected_representatives.values().data(),
+                        built.projected_representatives.values().size_bytes());
+            return k;
+        }
+        auto built = uh::detail::build_l2_hierarchical_projection_fp32(
+            m, std::move(sample), p, eng);
+        const int64_t k = copy_out(built);
+        std::memcpy(proj32, built.jl_matrix.values().data(),
+                    built.jl_matrix.values().size_bytes());
+        std::memcpy(refs32, built.projected_representatives.values().data(),
+                    built.projected_representatives.values().size_bytes());
+        return k;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_importance_probabilities(int metric_l1, const float* refs, uint64_t r,
+                                 uint64_t d, double* out)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        const auto p = metric_l1
+                           ? uh::compute_l1_importance_probabilities(
+                                 m, uh::ExecutionPolicy::sequential)
+                           : uh::compute_l2_importance_probabilities(
+                                 m, uh::ExecutionPolicy::sequential);
+        for (uint64_t i = 0; i < d; ++i) out[i] = p[i];
+    });
+}
+
+}  // extern "C"

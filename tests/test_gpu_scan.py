@@ -1,0 +1,175 @@
+"""GPU parity of the dense scan engine (exact L2 / L1, plain binary32 path)
+through the C ABI, against the oracle's emulation of the reference's CUDA
+kernels (bit-identical distances, hence bit-identical argmin) and the
+reference's own CUDA test fixtures (tests/exact/l{1,2}/cuda_exact_l*_test.cpp)."""
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+TILES = [(1, 8, 8, 8), (1, 12, 12, 8), (1, 16, 8, 4), (1, 8, 16, 2), (1, 4, 16, 1), (1, 2, 16, 1),
+         (1, 1, 16, 1), (4, 1, 8, 1), (4, 2, 8, 1), (4, 4, 8, 1), (4, 8, 4, 2)]
+
+
+def test_exact_l2_reference_fixture(uh):
+    # tests/exact/l2/cuda_exact_l2_index_test.cpp:83-150
+    refs = np.array([[0, 0], [5, 0], [2, 4]], F32)
+    index = uh.HipExactL2Index(refs)
+    ws = index.make_query_workspace(4)
+    assert index.device() == 0 and uh.device_name(0)
+    assert ws.maximum_batch_size() == 4
+    assert index.payload_bytes() == 6 * 4 + 3 * 4
+    q = np.array([1, 0, 3, 3, 2.5, 0, 5, 0], F32)
+    idx, dist = index.query_batch(q, 4, ws)
+    assert list(idx) == [0, 2, 0, 1]  # the tie (2.5, 0) resolves to row 0
+    assert list(dist) == [1.0, 2.0, 6.25, 0.0]
+    assert index.query(q[:2], ws) == 0
+
+
+def test_cancellation_fixture(uh):
+    # cuda_exact_l2_index_test.cpp:152-174: direct binary32 must pick row 1
+    base = F32(1048576.0)
+    index = uh.HipDenseL2ScanIndex(np.array([[base + F32(0.25)], [base + F32(0.125)]], F32))
+    ws = index.make_query_workspace(1)
+    idx, dist = index.query_batch(np.array([base], F32), 1, ws)
+    assert idx[0] == 1 and dist[0] == F32(0.015625)
+
+
+def test_error_paths(uh):
+    # cuda_exact_l2_index_test.cpp:176-246
+    refs = np.array([[0, 0], [5, 0], [2, 4]], F32)
+    index = uh.HipExactL2Index(refs)
+    ws = index.make_query_workspace(4)
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query_batch(np.zeros(10, F32), 5, ws)  # capacity
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query_batch(np.zeros(3, F32), 2, ws)  # dimensions
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query(np.array([np.nan, 0], F32), ws)  # non-finite query
+    other = uh.HipExactL2Index(refs)
+    with pytest.raises(uh.UhannInvalidArgument):
+        other.query(np.array([1, 0], F32), ws)  # foreign workspace
+    with pytest.raises(uh.UhannInvalidArgument):
+        uh.HipExactL2Index(np.array([[np.inf, 0]], F32))
+    with pytest.raises(uh.UhannInvalidArgument):
+        uh.HipExactL2Index(np.zeros((0, 3), F32))
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.make_query_workspace(0)
+    with pytest.raises(uh.UhannInvalidArgument):
+        uh.HipDenseL1ScanIndex(refs, np.array([-1.0, 1.0], F32))
+    # count == 0 is a no-op
+    idx, _ = index.query_batch(np.zeros(0, F32), 0, ws)
+    assert idx.size == 0
+
+
+def test_exact_l1_reference_fixture(uh):
+    # tests/exact/l1/cuda_exact_l1_index_test.cpp:98-135
+    refs = np.array([[0, 0], [5, 0], [2, 4]], F32)
+    index = uh.HipExactL1Index(refs)
+    ws = index.make_query_workspace(4)
+    assert index.payload_bytes() == 6 * 4
+    idx, _ = index.query_batch(np.array([1, 0, 3, 3, 2.5, 0, 5, 0], F32), 4, ws)
+    assert list(idx) == [0, 2, 0, 1]
+    weighted = uh.HipDenseL1ScanIndex(refs, np.array([0.5, 4.0], F32))
+    assert weighted.payload_bytes() == 6 * 4 + 2 * 4
+    wws = weighted.make_query_workspace(1)
+    assert weighted.query(np.array([3, 1], F32), wws) == 1
+
+
+def _data(seed, r, d, b, dup=True):
+    g = rng(seed)
+    refs = (g.random((r, d), dtype=np.float32) * 4).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 4).astype(F32)
+    if dup and r > 3:
+        refs[r - 1] = refs[1]  # exact duplicate: the lower index must win
+        q[0] = refs[1]
+    return refs, q
+
+
+@pytest.mark.parametrize(
+    "r,d,b",
+    [(37, 1000, 19), (5, 3, 7), (64, 256, 33), (130, 4097, 5), (1, 70, 3), (300, 60660, 2)],
+)
+@pytest.mark.parametrize("metric", ["l2", "l1", "l1w"])
+def test_parity_bit_exact(uh, oracle_mod, metric, r, d, b):
+    refs, q = _data(r * 31 + d, r, d, b)
+    if metric == "l2":
+        index = uh.HipDenseL2ScanIndex(refs)
+        expect = oracle_mod.k2_distances(refs, q)
+    elif metric == "l1":
+        index = uh.HipDenseL1ScanIndex(refs)
+        expect = oracle_mod.k5_distances(refs, q)
+    else:
+        w = (rng(d).random(d, dtype=np.float32) * 3 + 0.1).astype(F32)
+        index = uh.HipDenseL1ScanIndex(refs, w)
+        expect = oracle_mod.k5_distances(refs, q, w, 0)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(expect)
+    ws = index.make_query_workspace(b)
+    idx, dist = index.query_batch(q, b, ws)
+    np.testing.assert_array_equal(idx, exp_idx)
+    np.testing.assert_array_equal(dist.view(np.uint32), exp_dist.view(np.uint32))
+
+
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("d", [1000, 515, 4096])
+def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
+    import torch
+
+    r, b = 45, 13
+    refs, q = _data(7 + d, r, d, b)
+    index = uh.HipDenseL2ScanIndex(refs)
+    ws = index.make_query_workspace(b)
+    expect = oracle_mod.k2_distances(refs, q)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(expect)
+    dq = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    if tile[0] == 4 and d % 4 != 0:
+        with pytest.raises(uh.UhannInvalidArgument):
+            index.query_device_tile(dq, b, ws, oi, od, tile)
+        return
+    index.query_device_tile(dq, b, ws, oi, od, tile, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+
+
+def test_device_batch_and_async(uh, oracle_mod):
+    import torch
+
+    refs, q = _data(3, 50, 777, 20)
+    index = uh.HipDenseL1ScanIndex(refs)
+    ws = index.make_query_workspace(20)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k5_distances(refs, q))
+    dq = torch.from_numpy(q).cuda()
+    idx, dist = index.query_device_batch(dq, 20, ws)
+    np.testing.assert_array_equal(idx, exp_idx)
+    np.testing.assert_array_equal(dist, exp_dist)
+    # strided device queries (row stride > dim)
+    padded = torch.zeros((20, 800), dtype=torch.float32, device="cuda")
+    padded[:, :777] = dq
+    oi = torch.zeros(20, dtype=torch.int64, device="cuda")
+    od = torch.zeros(20, dtype=torch.float32, device="cuda")
+    index.query_device_async(padded, 20, ws, oi, od, stream=torch.cuda.current_stream(), ld_queries=800)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy(), exp_dist)
+
+
+def test_zero_dimension_and_overflow(uh, oracle_mod):
+    # dim == 0: every distance is 0 and row 0 wins (cuda_dense_l1_scan_index.cu:487-490)
+    index = uh.HipDenseL2ScanIndex(np.zeros((4, 0), F32))
+    ws = index.make_query_workspace(2)
+    idx, dist = index.query_batch(np.zeros(0, F32), 2, ws)
+    assert list(idx) == [0, 0] and list(dist) == [0.0, 0.0]
+    # huge finite values overflow to +inf; the lowest index among +inf wins
+    refs = np.array([[3e38, 3e38], [-3e38, -3e38], [3e38, -3e38]], F32)
+    index = uh.HipDenseL2ScanIndex(refs)
+    ws = index.make_query_workspace(1)
+    idx, dist = index.query_batch(np.array([-3e38, 3e38], F32), 1, ws)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, np.array([[-3e38, 3e38]], F32)))
+    assert idx[0] == exp_idx[0] and np.isinf(dist[0]) == np.isinf(exp_dist[0])

@@ -1,0 +1,134 @@
+"""ctypes binding of the engine's C ABI (include/uhann.h).
+
+The shared library is built in-tree (``ultrahigh_ann_amd/lib/libuhann.so``)
+by ``__graft_entry__.build()`` or ``make -C ultrahigh_ann_amd/csrc``. There is
+no fallback: if the library is missing every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int64, c_size_t, c_uint64, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libuhann.so")
+
+# Status codes of include/uhann.h and the Python exceptions standing in for
+# the reference's C++ exception types.
+OK = 0
+
+
+class UhannError(RuntimeError):
+    """std::runtime_error: device errors, no device."""
+
+
+class UhannInvalidArgument(ValueError):
+    """std::invalid_argument."""
+
+
+class UhannLengthError(ValueError):
+    """std::length_error."""
+
+
+class UhannOverflowError(ArithmeticError):
+    """std::overflow_error: non-finite results."""
+
+
+class UhannDomainError(ValueError):
+    """std::domain_error: invalid probabilities."""
+
+
+class UhannOutOfRange(IndexError):
+    """std::out_of_range."""
+
+
+class UhannLogicError(RuntimeError):
+    """std::logic_error."""
+
+
+_STATUS = {
+    1: UhannInvalidArgument,
+    2: UhannLengthError,
+    3: UhannError,
+    4: UhannOverflowError,
+    5: UhannDomainError,
+    6: UhannOutOfRange,
+    7: UhannLogicError,
+}
+
+_lib = None
+
+
+def _declare(lib):
+    u64p = POINTER(c_uint64)
+    f32p = POINTER(c_float)
+    f64p = POINTER(c_double)
+    sigs = {
+        "uhann_last_error": (c_char_p, []),
+        "uhann_device_available": (c_int, []),
+        "uhann_device_name": (c_int, [c_int, ctypes.c_char_p, c_size_t]),
+        "uhann_scan_create": (c_int, [c_int, f32p, c_uint64, c_uint64, f32p, c_int, POINTER(c_void_p)]),
+        "uhann_scan_destroy": (None, [c_void_p]),
+        "uhann_scan_info": (c_int, [c_void_p, u64p, u64p, u64p, POINTER(c_int)]),
+        "uhann_scan_workspace_create": (c_int, [c_void_p, c_uint64, POINTER(c_void_p)]),
+        "uhann_workspace_destroy": (None, [c_void_p]),
+        "uhann_workspace_info": (c_int, [c_void_p, u64p, u64p]),
+        "uhann_scan_query_batch": (c_int, [c_void_p, c_void_p, f32p, c_uint64, u64p, f32p, c_int]),
+        "uhann_scan_query_device_batch": (
+            c_int,
+            [c_void_p, c_void_p, c_void_p, c_uint64, u64p, f32p, c_int],
+        ),
+        "uhann_scan_query_device_async": (
+            c_int,
+            [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p, c_int, c_void_p],
+        ),
+        "uhann_scan_query_device_tile": (
+            c_int,
+            [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p, c_int, c_int,
+             c_int, c_int, c_void_p],
+        ),
+    }
+    optional = {}
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    for name, (res, args) in optional.items():
+        fn = getattr(lib, name, None)
+        if fn is not None:
+            fn.restype = res
+            fn.argtypes = args
+    _ = (f64p, c_int64)
+    return lib
+
+
+def lib():
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UhannError(
+                f"HIP engine library not built: {LIB_PATH} is missing "
+                "(run __graft_entry__.build() or make -C ultrahigh_ann_amd/csrc)"
+            )
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = lib().uhann_last_error().decode(errors="replace")
+        raise _STATUS.get(status, UhannError)(msg)
+
+
+def f32ptr(arr):
+    return arr.ctypes.data_as(POINTER(c_float)) if arr is not None else None
+
+
+def f64ptr(arr):
+    return arr.ctypes.data_as(POINTER(c_double)) if arr is not None else None
+
+
+def u64ptr(arr):
+    return arr.ctypes.data_as(POINTER(c_uint64)) if arr is not None else None

@@ -1,0 +1,61 @@
+// Launch interfaces of the HIP kernels (implemented in *.hip). Host code in
+// engine.cpp validates shapes before calling any of these; every launcher
+// checks the launch status and throws std::runtime_error on failure.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace uhann::kern {
+
+enum class Metric : int { l2 = 0, l1 = 1 };
+
+// One dense FP32 scan: `nq` query rows against `rows` reference rows, both
+// row-major with leading dimensions ld_q / ld_ref (in floats), `dim` scanned
+// coordinates. weights (L1 only) holds one float per coordinate or is null.
+// keys[q] receives atomicMin of ((distance bits) << 32 | row); it must hold
+// ~0 before the launch (fill_keys).
+struct ScanArgs {
+    const float* refs = nullptr;
+    uint64_t ld_ref = 0;
+    uint32_t rows = 0;
+    const float* queries = nullptr;
+    uint64_t ld_q = 0;
+    uint32_t nq = 0;
+    const float* weights = nullptr;
+    uint32_t dim = 0;
+    unsigned long long* keys = nullptr;
+};
+
+// Register tile of the direct scan: a block of 256/rpt threads, each thread
+// owning `rpt` consecutive residues of the 256-column stride, accumulating a
+// tq x tr tile of (query, reference) pairs. group_q q-tiles share one sweep
+// over r-tiles (L2 locality). weight_fma selects fmaf(w,|d|,acc) for
+// weighted L1 (else rounded product, then add).
+struct ScanTile {
+    int rpt = 1;
+    int tq = 8;
+    int tr = 8;
+    int group_q = 8;
+};
+
+bool scan_tile_supported(const ScanTile& tile);
+void fill_keys(unsigned long long* keys, uint32_t n, hipStream_t stream);
+void scan_direct(Metric metric, bool weight_fma, const ScanArgs& args,
+                 const ScanTile& tile, hipStream_t stream);
+void finalize_keys(const unsigned long long* keys, uint32_t n, uint64_t* out_idx,
+                   float* out_dist, hipStream_t stream);
+
+// Gather of sampled coordinates: out[q][j] = src[q][cols[j]] (* scales[j]
+// when scales is non-null), for q < nq, j < k. Sets *invalid to 1 when a
+// gathered (or scaled) value is not finite.
+void gather_columns(const float* src, uint64_t ld_src, uint32_t nq,
+                    const uint32_t* cols, const float* scales, uint32_t k,
+                    float* out, uint64_t ld_out, int* invalid, hipStream_t stream);
+
+// Elementwise finiteness check over n floats; sets *invalid to 1 on a
+// non-finite value.
+void check_finite(const float* values, uint64_t n, int* invalid, hipStream_t stream);
+
+}  // namespace uhann::kern
