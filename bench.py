@@ -8,6 +8,11 @@ one batch through the engine's C ABI with queries and references already
 resident in HBM. The timed region is bracketed by torch.cuda.synchronize()
 and the per-step latency comes from HIP events on the launch stream.
 
+`value` is the mixed-precision strategy (BF16 Gram on MFMA, exact binary32
+re-rank of a bounded candidate set, binary64 near ties); the plain binary32
+direct path is timed beside it on the same batch, and both are checked
+against the binary64 reference CPU path on a query sample.
+
 Prints ONE JSON line (metric, value = queries/s, roofline of the dominant
 kernel, CPU baseline timed on this host with the reference's CPU path).
 """
@@ -40,7 +45,7 @@ def cpu_threads() -> int:
     return max(1, min(16, os.cpu_count() or 1))
 
 
-def cpu_baseline(refs_np, queries_np, sample, gpu_idx):
+def cpu_baseline(refs_np, queries_np, sample):
     """Reference CPU exact L2 (oracle/_ref, OpenMP parallel_queries) on a
     bounded query sample; also the binary64 argmin for agreement."""
     import ctypes
@@ -65,7 +70,6 @@ def cpu_baseline(refs_np, queries_np, sample, gpu_idx):
                                       sample, threads, oracle._p(idx, oracle._u64p),
                                       ctypes.POINTER(ctypes.c_double)())
     elapsed = time.perf_counter() - t0
-    agree = float(np.mean(idx == gpu_idx[:sample]))
     return {
         "value": sample / elapsed,
         "unit": "queries/s",
@@ -73,7 +77,7 @@ def cpu_baseline(refs_np, queries_np, sample, gpu_idx):
         "kind": kind,
         "sample": f"{sample} queries of the same batch, exact L2 binary64 over r={r} d={d}, "
                   f"OpenMP parallel_queries, {elapsed:.2f} s wall",
-    }, agree
+    }, idx
 
 
 def main():
@@ -107,45 +111,58 @@ def main():
     log(f"generating synthetic data r={r} d={d} B={B} seed={args.seed}")
     refs, queries = synthetic.log_expression(r, B, d, seed=args.seed, device="cuda")
     refs_np = refs.cpu().numpy()
-    log("building index (upload references)")
+    log("building index (upload references, BF16 copy)")
     index = uh.HipExactL2Index(refs_np)
+    index.enable_mixed()
     ws = index.make_query_workspace(B)
     stream = torch.cuda.current_stream()
     out_idx = torch.zeros(B, dtype=torch.int64, device="cuda")
     out_dist = torch.zeros(B, dtype=torch.float32, device="cuda")
 
-    def step():
-        index.query_device_async(queries, B, ws, out_idx, out_dist, stream=stream)
-
-    for i in range(args.warmup):
-        step()
+    def run(strategy, steps, warmup, label):
+        for i in range(warmup):
+            index.query_device_async(queries, B, ws, out_idx, out_dist, strategy=strategy, stream=stream)
+            torch.cuda.synchronize()
+        events = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
         torch.cuda.synchronize()
-        log(f"warmup {i + 1}/{args.warmup} done")
+        t0 = time.perf_counter()
+        events[0].record(stream)
+        for i in range(steps):
+            index.query_device_async(queries, B, ws, out_idx, out_dist, strategy=strategy, stream=stream)
+            events[i + 1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = np.array([events[i].elapsed_time(events[i + 1]) for i in range(steps)])
+        log(f"{label}: {steps} steps {wall:.3f} s wall, per-step ms {np.round(ms, 3).tolist()}")
+        return wall, ms, out_idx.cpu().numpy().astype(np.uint64)
 
+    fp32_steps = max(1, min(args.steps, 3))
+    fp32_wall, fp32_ms, fp32_idx = run(uh.STRATEGY_DIRECT, fp32_steps, 1, "fp32 direct")
     K = args.steps
-    events = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    events[0].record(stream)
-    for i in range(K):
-        step()
-        events[i + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_ms = np.array([events[i].elapsed_time(events[i + 1]) for i in range(K)])
-    log(f"timed {K} steps: {wall:.3f} s wall, per-step ms {np.round(step_ms, 3).tolist()}")
-
-    gpu_idx = out_idx.cpu().numpy().astype(np.uint64)
+    wall, step_ms, mixed_idx = run(uh.STRATEGY_GRAM_BF16, K, args.warmup, "bf16 mixed")
+    stats = ws.mixed_stats()
     qps = B * K / wall
-    kernel_ms = float(step_ms.mean())
-    flops = 3.0 * r * d * B
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    fp32_qps = B * fp32_steps / fp32_wall
+
+    # Per-stage HIP-event times of the mixed strategy (same stream), outside
+    # the timed region.
+    ws.set_timing(True)
+    stage = []
+    for _ in range(3):
+        index.query_device_async(queries, B, ws, out_idx, out_dist, strategy=uh.STRATEGY_GRAM_BF16,
+                                 stream=stream)
+        stage.append(ws.stage_times())
+    ws.set_timing(False)
+    stage = np.array(stage).mean(0)
+    gemm_ms = float(stage[1])
+    flops = 2.0 * B * r * d
+    achieved = flops / (gemm_ms * 1e-3) / 1e12
     roofline = {
         "bound": "mfma",
-        "achieved": round(achieved, 3),
-        "peak": PEAK["fp32"],
+        "achieved": round(achieved, 2),
+        "peak": PEAK["bf16"],
         "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK["fp32"], 4),
+        "frac": round(achieved / PEAK["bf16"], 4),
         "traffic": None,
     }
 
@@ -155,8 +172,13 @@ def main():
         threads = cpu_threads()
         sample = args.cpu_sample or max(8, 2 * threads)
         log(f"CPU baseline: {sample} queries on {threads} threads")
-        cpu, agreement = cpu_baseline(refs_np, queries.cpu().numpy(), sample, gpu_idx)
-        log(f"CPU baseline {cpu['value']:.2f} q/s; FP32-vs-FP64 argmin agreement {agreement:.4f}")
+        cpu, fp64_idx = cpu_baseline(refs_np, queries.cpu().numpy(), sample)
+        agreement = {
+            "sample": sample,
+            "fp32_direct": float(np.mean(fp32_idx[:sample] == fp64_idx)),
+            "bf16_mixed": float(np.mean(mixed_idx[:sample] == fp64_idx)),
+        }
+        log(f"CPU baseline {cpu['value']:.2f} q/s; agreement with binary64 {agreement}")
 
     line = {
         "metric": METRIC,
@@ -169,18 +191,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16",
         "data": "synthetic",
         "config": {
-            "workload": f"exact-l2 plain binary32 direct scan, r={r} d={d} batch={B}",
+            "workload": f"exact-l2 r={r} d={d} batch={B}: BF16 Gram on MFMA + exact re-rank",
             "references": r,
             "dim": d,
             "batch": B,
-            "strategy": "direct-fp32",
+            "strategy": "gram-bf16 (fp32 re-rank, fp64 near ties)",
         },
         "p50_ms": round(float(np.percentile(step_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 3),
-        "agreement_fp64_sample": agreement,
+        "fp32_direct": {
+            "value": round(fp32_qps, 2),
+            "unit": "queries/s",
+            "ms_per_step": round(fp32_wall * 1e3 / fp32_steps, 3),
+            "speedup_mixed_vs_fp32": round(qps / fp32_qps, 2),
+        },
+        "mixed_stages_ms": {"bf16_convert": round(float(stage[0]), 3), "gram_gemm": round(gemm_ms, 3),
+                            "select_rerank": round(float(stage[2]), 3),
+                            "escalation": round(float(stage[3]), 3)},
+        "mixed_counters": {"avg_candidates": round(stats["candidates"] / B, 3),
+                           "fp64_reranked": stats["fp64_reranked"], "escalated": stats["escalated"]},
+        "agreement_fp64": agreement,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

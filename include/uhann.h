@@ -44,6 +44,12 @@ extern "C" {
  * reference's direct CUDA kernels (CudaDenseL2QueryStrategy::direct,
  * cuda_dense_l2_scan_index.hpp:17-23). */
 #define UHANN_STRATEGY_DIRECT 0
+/* L2 only. ||q||^2 + ||v||^2 - 2 q.v with the inner product in BF16 on MFMA
+ * (FP32 accumulation), a per-query candidate set bounded by the BF16 error,
+ * an exact re-rank of the candidates in the direct path's binary32 order and
+ * a binary64 re-rank of binary32 near ties. New (the reference's GEMM
+ * strategy, cuda_dense_l2_scan_index.hpp:21-22, is unrefined FP32 SGEMM). */
+#define UHANN_STRATEGY_GRAM_BF16 1
 
 /* Index kinds reported by uhann_index_info. */
 #define UHANN_KIND_EXACT 0
@@ -125,6 +131,26 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
                                  const float* device_queries, uint64_t ld_queries, uint64_t count,
                                  uint64_t* device_out_idx, float* device_out_dist, int rpt, int tq,
                                  int tr, int group_q, void* stream);
+
+/* Builds the BF16 reference copy and row statistics the GRAM_BF16 strategy
+ * needs (rows x dim bf16 + 16 bytes per row of device memory). Call before
+ * sharing the handle between threads. L2 only. No counterpart in the
+ * reference (its GEMM strategy needs no preparation beyond K1 norms,
+ * cuda_dense_l2_scan_index.cu:449-462). */
+int uhann_scan_enable_mixed(uhann_scan* scan, int strategy);
+
+/* Counters of the last GRAM_BF16 batch on this workspace: total candidate
+ * set size, queries decided in binary64, queries escalated to the direct
+ * scan. Synchronises the workspace stream. */
+int uhann_workspace_mixed_stats(uhann_workspace* ws, uint64_t* candidates, uint64_t* fp64_reranked,
+                                uint64_t* escalated);
+
+/* Stage timing of GRAM_BF16 batches on this workspace with HIP events on the
+ * launch stream (enable != 0 turns it on; each timed batch then waits for its
+ * last event). uhann_workspace_stage_times copies up to n stage durations in
+ * milliseconds: bf16 conversion, Gram GEMM, select + re-rank, escalation. */
+int uhann_workspace_set_timing(uhann_workspace* ws, int enable);
+int uhann_workspace_stage_times(const uhann_workspace* ws, float* ms, int n);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,126 @@
+"""GPU checks of the mixed-precision exact L2 strategy (BF16 Gram on MFMA +
+bounded candidate set + exact re-rank + binary64 near ties).
+
+Bar: every result is the binary32 direct path's result (bit-identical
+distance), except where the binary64 re-rank chose another row, in which case
+that row's binary64 distance is not larger; and the result agrees with the
+binary64 oracle at least as often as the binary32 path does. The reference's
+cancellation fixture (tests/exact/l2/cuda_exact_l2_index_test.cpp:152-174),
+where its unrefined GEMM strategy returns the wrong row, must return the right
+one."""
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def _check_against_direct(uh, oracle_mod, refs, q):
+    import torch
+
+    b = q.shape[0]
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed()
+    ws = index.make_query_workspace(b)
+    d_idx, d_dist = index.query_batch(q, b, ws, uh.STRATEGY_DIRECT)
+    m_idx, m_dist = index.query_batch(q, b, ws, uh.STRATEGY_GRAM_BF16)
+    stats = ws.mixed_stats()
+    # device-query path gives the same answer
+    dm_idx, dm_dist = index.query_device_batch(torch.from_numpy(q).cuda(), b, ws, uh.STRATEGY_GRAM_BF16)
+    np.testing.assert_array_equal(dm_idx, m_idx)
+    np.testing.assert_array_equal(dm_dist, m_dist)
+    k2 = oracle_mod.k2_distances(refs, q)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(k2)
+    np.testing.assert_array_equal(d_idx, exp_idx)
+    table = oracle_mod.fp64_table(False, refs, q)
+    fp64_idx = table.argmin(1)
+    rows = np.arange(b)
+    diff = m_idx != d_idx
+    # same row -> same binary32 distance bit for bit
+    np.testing.assert_array_equal(m_dist[~diff].view(np.uint32), d_dist[~diff].view(np.uint32))
+    # mixed distance is always the binary32 (reference-order) distance of its row
+    np.testing.assert_array_equal(m_dist.view(np.uint32), k2[rows, m_idx.astype(np.int64)].view(np.uint32))
+    # a different row only through the binary64 near-tie re-rank, never worse in binary64
+    assert np.all(table[rows[diff], m_idx[diff].astype(np.int64)] <= table[rows[diff], d_idx[diff].astype(np.int64)])
+    agree_mixed = np.mean(m_idx == fp64_idx)
+    agree_fp32 = np.mean(d_idx == fp64_idx)
+    assert agree_mixed >= agree_fp32
+    return stats, agree_mixed, agree_fp32
+
+
+@pytest.mark.parametrize("r,d,b", [(300, 3000, 40), (129, 257, 130), (1000, 8192, 64), (7, 65, 3)])
+def test_mixed_random(uh, oracle_mod, r, d, b):
+    g = rng(r + d)
+    refs = (g.random((r, d), dtype=np.float32) * 4).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 4).astype(F32)
+    refs[-1] = refs[0]  # duplicate rows: the lower index must win
+    q[0] = refs[0]
+    stats, _, _ = _check_against_direct(uh, oracle_mod, refs, q)
+    assert stats["escalated"] <= b
+
+
+def test_mixed_log_expression(uh, oracle_mod):
+    import torch
+
+    from ultrahigh_ann_amd import synthetic
+
+    refs, q = synthetic.log_expression(700, 96, 20000, seed=11, device="cuda")
+    refs, q = refs.cpu().numpy(), q.cpu().numpy()
+    stats, agree_m, agree_f = _check_against_direct(uh, oracle_mod, refs, q)
+    # the candidate sets stay small on this data
+    assert stats["escalated"] == 0
+    assert stats["candidates"] <= 16 * 96
+    assert agree_m == 1.0
+    del torch
+
+
+def test_mixed_cancellation_fixture(uh):
+    base = F32(1048576.0)
+    index = uh.HipDenseL2ScanIndex(np.array([[base + F32(0.25)], [base + F32(0.125)]], F32))
+    index.enable_mixed()
+    ws = index.make_query_workspace(1)
+    idx, dist = index.query_batch(np.array([base], F32), 1, ws, uh.STRATEGY_GRAM_BF16)
+    assert idx[0] == 1 and dist[0] == F32(0.015625)
+
+
+def test_mixed_escalation_to_direct_scan(uh, oracle_mod):
+    # 100 identical rows: every one is a candidate (> 64) -> the query is
+    # answered by the exact direct scan, which returns the lowest row.
+    g = rng(5)
+    base = (g.random(512, dtype=np.float32) * 2).astype(F32)
+    refs = np.tile(base, (100, 1))
+    refs[37] += 0.5
+    q = np.stack([base + F32(0.01), base]).astype(F32)
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed()
+    ws = index.make_query_workspace(2)
+    idx, dist = index.query_batch(q, 2, ws, uh.STRATEGY_GRAM_BF16)
+    assert ws.mixed_stats()["escalated"] == 2
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q))
+    np.testing.assert_array_equal(idx, exp_idx)
+    np.testing.assert_array_equal(dist, exp_dist)
+
+
+def test_mixed_requires_enable_and_l2(uh):
+    refs = np.ones((3, 4), F32)
+    index = uh.HipExactL2Index(refs)
+    ws = index.make_query_workspace(1)
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query_batch(np.ones(4, F32), 1, ws, uh.STRATEGY_GRAM_BF16)
+    with pytest.raises(uh.UhannInvalidArgument):
+        uh.HipExactL1Index(refs).enable_mixed()
+
+
+def test_mixed_stage_timing(uh):
+    g = rng(9)
+    refs = (g.random((256, 1024), dtype=np.float32)).astype(F32)
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed()
+    ws = index.make_query_workspace(16)
+    ws.set_timing(True)
+    index.query_batch(refs[:16].copy(), 16, ws, uh.STRATEGY_GRAM_BF16)
+    t = ws.stage_times()
+    assert len(t) == 4 and all(x >= 0 for x in t) and t[1] > 0

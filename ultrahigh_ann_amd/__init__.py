@@ -39,11 +39,13 @@ from ._native import (
 METRIC_L2 = 0
 METRIC_L1 = 1
 STRATEGY_DIRECT = 0
+STRATEGY_GRAM_BF16 = 1
 
 __all__ = [
     "METRIC_L1",
     "METRIC_L2",
     "STRATEGY_DIRECT",
+    "STRATEGY_GRAM_BF16",
     "HipDenseL1ScanIndex",
     "HipDenseL2ScanIndex",
     "HipExactL1Index",
@@ -104,6 +106,20 @@ class QueryWorkspace:
         check(_native.lib().uhann_workspace_info(self._h, None, ctypes.byref(pb)))
         return pb.value
 
+    def set_timing(self, enable: bool = True):
+        check(_native.lib().uhann_workspace_set_timing(self._h, int(enable)))
+
+    def stage_times(self) -> list:
+        ms = np.zeros(4, np.float32)
+        check(_native.lib().uhann_workspace_stage_times(self._h, f32ptr(ms), 4))
+        return [float(x) for x in ms]
+
+    def mixed_stats(self) -> dict:
+        c, f, e = c_uint64(), c_uint64(), c_uint64()
+        check(_native.lib().uhann_workspace_mixed_stats(self._h, ctypes.byref(c), ctypes.byref(f),
+                                                          ctypes.byref(e)))
+        return {"candidates": c.value, "fp64_reranked": f.value, "escalated": e.value}
+
     def close(self):
         if self._h:
             _native.lib().uhann_workspace_destroy(self._h)
@@ -160,6 +176,10 @@ class _DenseScan:
         d = c_int()
         check(_native.lib().uhann_scan_info(self._h, None, None, None, ctypes.byref(d)))
         return d.value
+
+    def enable_mixed(self, strategy=STRATEGY_GRAM_BF16):
+        """Prepare the BF16 reference copy for the mixed-precision strategy."""
+        check(_native.lib().uhann_scan_enable_mixed(self._h, strategy))
 
     def make_query_workspace(self, maximum_batch_size: int) -> QueryWorkspace:
         h = c_void_p()

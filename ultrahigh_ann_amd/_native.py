@@ -88,6 +88,10 @@ def _declare(lib):
             [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p, c_int, c_int,
              c_int, c_int, c_void_p],
         ),
+        "uhann_scan_enable_mixed": (c_int, [c_void_p, c_int]),
+        "uhann_workspace_mixed_stats": (c_int, [c_void_p, u64p, u64p, u64p]),
+        "uhann_workspace_set_timing": (c_int, [c_void_p, c_int]),
+        "uhann_workspace_stage_times": (c_int, [c_void_p, f32p, c_int]),
     }
     optional = {}
     for name, (res, args) in sigs.items():

@@ -129,8 +129,20 @@ Workspace::Workspace(const void* owner_value, int device_value, uint64_t max_bat
                     out_dist.bytes() + invalid.bytes();
 }
 
+void Workspace::mark(int i, hipStream_t s)
+{
+    if (!timing) return;
+    if (events[i] == nullptr) {
+        check_hip(hipEventCreate(&events[i]), "cannot create a HIP event");
+    }
+    check_hip(hipEventRecord(events[i], s), "cannot record a HIP event");
+}
+
 Workspace::~Workspace()
 {
+    for (hipEvent_t& e : events) {
+        if (e != nullptr) static_cast<void>(hipEventDestroy(e));
+    }
     if (stream != nullptr) {
         static_cast<void>(hipSetDevice(device));
         static_cast<void>(hipStreamSynchronize(stream));
@@ -157,7 +169,7 @@ kern::ScanTile choose_tile(uint64_t nq, bool aligned4)
 
 void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t count,
                  unsigned long long* keys, uint64_t* out_idx, float* out_dist,
-                 hipStream_t stream, const kern::ScanTile* forced_tile)
+                 hipStream_t stream, bool queries_padded, const kern::ScanTile* forced_tile)
 {
     kern::ScanArgs a;
     a.refs = scan.refs.data();
@@ -169,11 +181,16 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     a.weights = scan.weighted ? scan.weights.data() : nullptr;
     a.dim = static_cast<uint32_t>(scan.dim);
     a.keys = keys;
+    // 16-byte vector loads need aligned rows, and a row whose length is not a
+    // multiple of 4 must be padded (the last vector reads up to 3 floats past
+    // dim): true for workspace-staged queries, not assumed for caller memory.
     const bool aligned4 = (ld_q % 4U == 0) && (scan.ld % 4U == 0) &&
-                          (reinterpret_cast<uintptr_t>(queries) % 16U == 0);
+                          (reinterpret_cast<uintptr_t>(queries) % 16U == 0) &&
+                          (scan.dim % 4U == 0 || queries_padded);
     kern::ScanTile tile = forced_tile != nullptr ? *forced_tile : choose_tile(count, aligned4);
     if (tile.rpt > 1 && !aligned4) {
-        throw std::invalid_argument("vector scan tile needs 16-byte aligned query rows");
+        throw std::invalid_argument(
+            "vector scan tile needs 16-byte aligned, padded query rows");
     }
     if (a.dim == 0) {
         // No coordinates: every distance is zero and the reference returns row 0.
@@ -184,6 +201,111 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     kern::scan_direct(scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1,
                       /*weight_fma=*/true, a, tile, stream);
     kern::finalize_keys(keys, a.nq, out_idx, out_dist, stream);
+}
+
+
+void Scan::enable_mixed()
+{
+    if (metric != UHANN_METRIC_L2) {
+        throw std::invalid_argument("the mixed-precision strategy applies to the L2 metric only");
+    }
+    if (gram) return;
+    select_device(device);
+    auto g = std::make_unique<GramRefs>();
+    const uint64_t step = kern::gram_k_padding();
+    g->ldk = std::max<uint64_t>(step, (dim + step - 1) / step * step);
+    g->refs16 = DeviceBuffer<__bf16>(device, checked_mul(rows, g->ldk, "bf16 references"));
+    g->stats = DeviceBuffer<kern::RowStats>(device, rows);
+    kern::to_bf16_rows(refs.data(), ld, static_cast<uint32_t>(rows), static_cast<uint32_t>(dim),
+                       g->refs16.data(), g->ldk, g->stats.data(), nullptr);
+    check_hip(hipDeviceSynchronize(), "bf16 reference conversion failed");
+    payload_bytes += g->refs16.bytes() + g->stats.bytes();
+    gram = std::move(g);
+}
+
+void Workspace::reserve_mixed(const Scan& scan)
+{
+    if (approx.bytes() != 0 || scan.gram == nullptr) return;
+    const uint64_t mt = kern::gram_m_tile();
+    const uint64_t mrows = (max_batch + mt - 1) / mt * mt;
+    q16 = DeviceBuffer<__bf16>(device, checked_mul(mrows, scan.gram->ldk, "bf16 query workspace"));
+    qstats = DeviceBuffer<kern::RowStats>(device, mrows);
+    approx = DeviceBuffer<float>(device, checked_mul(max_batch, scan.rows, "Gram workspace"));
+    counters = DeviceBuffer<unsigned int>(device, 8);
+    escalate_list = DeviceBuffer<unsigned int>(device, max_batch);
+    esc_queries = DeviceBuffer<float>(device, checked_mul(max_batch, scan.ld, "escalation workspace"));
+    esc_idx = DeviceBuffer<uint64_t>(device, max_batch);
+    esc_dist = DeviceBuffer<float>(device, max_batch);
+    payload_bytes += q16.bytes() + qstats.bytes() + approx.bytes() + counters.bytes() +
+                     escalate_list.bytes() + esc_queries.bytes() + esc_idx.bytes() + esc_dist.bytes();
+}
+
+void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q,
+                  uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream)
+{
+    if (scan.gram == nullptr) {
+        throw std::invalid_argument("mixed-precision strategy not enabled for this index");
+    }
+    if (out_idx == nullptr || out_dist == nullptr) {
+        throw std::invalid_argument("the mixed-precision strategy needs index and distance outputs");
+    }
+    ws.reserve_mixed(scan);
+    const GramRefs& g = *scan.gram;
+    const uint32_t n = static_cast<uint32_t>(count);
+    ws.mark(0, stream);
+    kern::to_bf16_rows(queries, ld_q, n, static_cast<uint32_t>(scan.dim), ws.q16.data(), g.ldk,
+                       ws.qstats.data(), stream);
+    ws.mark(1, stream);
+    kern::gram_bf16(ws.q16.data(), g.ldk, n, g.refs16.data(), g.ldk, static_cast<uint32_t>(scan.rows),
+                    static_cast<uint32_t>(g.ldk), ws.qstats.data(), g.stats.data(), ws.approx.data(),
+                    scan.rows, stream);
+    ws.mark(2, stream);
+    kern::zero_u32(ws.counters.data(), 8, stream);
+    kern::SelectArgs a;
+    a.nq = n;
+    a.rows = static_cast<uint32_t>(scan.rows);
+    a.dim = static_cast<uint32_t>(scan.dim);
+    a.approx = ws.approx.data();
+    a.ld_approx = scan.rows;
+    a.qstats = ws.qstats.data();
+    a.vstats = g.stats.data();
+    a.acc_steps_sqrt = static_cast<float>(std::sqrt(static_cast<double>((g.ldk + 15) / 16)));
+    a.queries = queries;
+    a.ld_q = ld_q;
+    a.refs = scan.refs.data();
+    a.ld_ref = scan.ld;
+    const double k = std::ceil(static_cast<double>(scan.dim) / 256.0) + 10.0;
+    const double u = std::ldexp(1.0, -24);
+    const double gamma = k * u / (1.0 - k * u);
+    a.fp32_band = (1.0 + gamma) / (1.0 - gamma);
+    a.out_idx = out_idx;
+    a.out_dist = out_dist;
+    a.escalate_count = ws.counters.data();
+    a.escalate_list = ws.escalate_list.data();
+    a.stats = reinterpret_cast<kern::MixedStats*>(ws.counters.data() + 4);
+    kern::select_rerank(a, stream);
+    ws.mark(3, stream);
+    unsigned int escalated = 0;
+    check_hip(hipMemcpyAsync(&escalated, ws.counters.data(), sizeof(escalated), hipMemcpyDeviceToHost,
+                             stream),
+              "cannot read the escalation count");
+    check_hip(hipStreamSynchronize(stream), "mixed-precision query failed");
+    if (escalated > 0) {
+        kern::gather_rows(queries, ld_q, ws.escalate_list.data(), escalated,
+                          static_cast<uint32_t>(scan.dim), ws.esc_queries.data(), scan.ld, stream);
+        scan_launch(scan, ws.esc_queries.data(), scan.ld, escalated, ws.keys.data(), ws.esc_idx.data(),
+                    ws.esc_dist.data(), stream, true, nullptr);
+        kern::scatter_results(ws.esc_idx.data(), ws.esc_dist.data(), ws.escalate_list.data(), escalated,
+                              out_idx, out_dist, stream);
+    }
+    ws.mark(4, stream);
+    if (ws.timing) {
+        check_hip(hipEventSynchronize(ws.events[4]), "HIP event wait failed");
+        for (int i = 0; i < Workspace::kStages; ++i) {
+            check_hip(hipEventElapsedTime(&ws.stage_ms[i], ws.events[i], ws.events[i + 1]),
+                      "cannot read HIP event time");
+        }
+    }
 }
 
 }  // namespace uhann
@@ -209,6 +331,21 @@ uhann_workspace* checked_ws(const void* owner, uhann_workspace* ws, uint64_t cou
         throw std::invalid_argument("query count exceeds workspace capacity");
     }
     return ws;
+}
+
+// Runs one batch with `strategy` on the workspace stream/outputs.
+void run_strategy(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, bool padded,
+                  uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream,
+                  int strategy)
+{
+    if (strategy == UHANN_STRATEGY_DIRECT) {
+        scan_launch(scan, queries, ld_q, count, ws.keys.data(), out_idx, out_dist, stream, padded,
+                    nullptr);
+    } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
+        mixed_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
+    } else {
+        throw std::invalid_argument("unknown query strategy");
+    }
 }
 
 void copy_results(Workspace& ws, uint64_t count, uint64_t* out_idx, float* out_dist)
@@ -303,7 +440,6 @@ int uhann_scan_query_batch(const uhann_scan* scan, uhann_workspace* ws, const fl
     return guarded([&] {
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
         checked_ws(scan, ws, count);
-        if (strategy != UHANN_STRATEGY_DIRECT) throw std::invalid_argument("unknown strategy");
         const std::size_t n = checked_mul(count, scan->dim, "query batch");
         if (n != 0 && host_queries == nullptr) throw std::invalid_argument("queries must not be null");
         validate_finite_host(host_queries, n, "queries");
@@ -315,8 +451,8 @@ int uhann_scan_query_batch(const uhann_scan* scan, uhann_workspace* ws, const fl
                                        hipMemcpyHostToDevice, ws->stream),
                       "cannot copy queries to the HIP device");
         }
-        scan_launch(*scan, ws->queries.data(), scan->ld, count, ws->keys.data(), ws->out_idx.data(),
-                    ws->out_dist.data(), ws->stream, nullptr);
+        run_strategy(*scan, *ws, ws->queries.data(), scan->ld, true, count, ws->out_idx.data(),
+                     ws->out_dist.data(), ws->stream, strategy);
         copy_results(*ws, count, out_idx, out_dist);
     });
 }
@@ -328,14 +464,13 @@ int uhann_scan_query_device_batch(const uhann_scan* scan, uhann_workspace* ws,
     return guarded([&] {
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
         checked_ws(scan, ws, count);
-        if (strategy != UHANN_STRATEGY_DIRECT) throw std::invalid_argument("unknown strategy");
         if (count == 0) return;
         if (scan->dim != 0 && device_queries == nullptr) {
             throw std::invalid_argument("device query pointer must not be null");
         }
         select_device(scan->device);
-        scan_launch(*scan, device_queries, scan->dim, count, ws->keys.data(), ws->out_idx.data(),
-                    ws->out_dist.data(), ws->stream, nullptr);
+        run_strategy(*scan, *ws, device_queries, scan->dim, false, count, ws->out_idx.data(),
+                     ws->out_dist.data(), ws->stream, strategy);
         copy_results(*ws, count, out_idx, out_dist);
     });
 }
@@ -348,15 +483,14 @@ int uhann_scan_query_device_async(const uhann_scan* scan, uhann_workspace* ws,
     return guarded([&] {
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
         checked_ws(scan, ws, count);
-        if (strategy != UHANN_STRATEGY_DIRECT) throw std::invalid_argument("unknown strategy");
         if (count == 0) return;
         if (scan->dim != 0 && device_queries == nullptr) {
             throw std::invalid_argument("device query pointer must not be null");
         }
         if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
         select_device(scan->device);
-        scan_launch(*scan, device_queries, ld_queries, count, ws->keys.data(), device_out_idx,
-                    device_out_dist, static_cast<hipStream_t>(stream), nullptr);
+        run_strategy(*scan, *ws, device_queries, ld_queries, false, count, device_out_idx,
+                     device_out_dist, static_cast<hipStream_t>(stream), strategy);
     });
 }
 
@@ -377,7 +511,50 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
         if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
         select_device(scan->device);
         scan_launch(*scan, device_queries, ld_queries, count, ws->keys.data(), device_out_idx,
-                    device_out_dist, static_cast<hipStream_t>(stream), &tile);
+                    device_out_dist, static_cast<hipStream_t>(stream), false, &tile);
+    });
+}
+
+int uhann_scan_enable_mixed(uhann_scan* scan, int strategy)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        if (strategy != UHANN_STRATEGY_GRAM_BF16) throw std::invalid_argument("unknown mixed strategy");
+        scan->enable_mixed();
+    });
+}
+
+int uhann_workspace_mixed_stats(uhann_workspace* ws, uint64_t* candidates, uint64_t* fp64_reranked,
+                                uint64_t* escalated)
+{
+    return guarded([&] {
+        if (ws == nullptr) throw std::invalid_argument("workspace handle must not be null");
+        unsigned int host[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (ws->counters.bytes() != 0) {
+            select_device(ws->device);
+            check_hip(hipStreamSynchronize(ws->stream), "workspace stream failed");
+            check_hip(hipMemcpy(host, ws->counters.data(), sizeof(host), hipMemcpyDeviceToHost),
+                      "cannot read mixed-precision statistics");
+        }
+        if (candidates) *candidates = host[4];
+        if (fp64_reranked) *fp64_reranked = host[5];
+        if (escalated) *escalated = host[6];
+    });
+}
+
+int uhann_workspace_set_timing(uhann_workspace* ws, int enable)
+{
+    return guarded([&] {
+        if (ws == nullptr) throw std::invalid_argument("workspace handle must not be null");
+        ws->timing = enable != 0;
+    });
+}
+
+int uhann_workspace_stage_times(const uhann_workspace* ws, float* ms, int n)
+{
+    return guarded([&] {
+        if (ws == nullptr || ms == nullptr) throw std::invalid_argument("null argument");
+        for (int i = 0; i < n && i < Workspace::kStages; ++i) ms[i] = ws->stage_ms[i];
     });
 }
 

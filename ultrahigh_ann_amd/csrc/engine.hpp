@@ -7,7 +7,10 @@
 #include <cstdint>
 #include <string>
 
+#include <memory>
+
 #include "common.hpp"
+#include "gram.hpp"
 #include "kernels.hpp"
 
 namespace uhann {
@@ -15,6 +18,14 @@ namespace uhann {
 const char* last_error() noexcept;
 uint64_t padded_ld(uint64_t dim);
 void validate_finite_host(const float* values, std::size_t n, const char* what);
+
+// BF16 copy of the references for the mixed-precision strategy: rows x ldk
+// (ldk = dim rounded up to the Gram K step, zero padded) plus row statistics.
+struct GramRefs {
+    uint64_t ldk = 0;
+    DeviceBuffer<__bf16> refs16;
+    DeviceBuffer<kern::RowStats> stats;
+};
 
 // Dense FP32 references resident on one device, row-major with a leading
 // dimension padded to a multiple of 4 floats.
@@ -34,6 +45,9 @@ struct Scan {
     DeviceBuffer<float> weights;
     uint64_t payload_bytes = 0;
     std::string device_name;
+    std::unique_ptr<GramRefs> gram;  // set by enable_mixed()
+
+    void enable_mixed();
 };
 
 // Per-caller device buffers and a private stream. `owner` identifies the
@@ -57,14 +71,40 @@ struct Workspace {
     DeviceBuffer<int> invalid;
     hipStream_t stream = nullptr;
     uint64_t payload_bytes = 0;
+
+    // Mixed-precision buffers, allocated on the first mixed query.
+    DeviceBuffer<__bf16> q16;
+    DeviceBuffer<kern::RowStats> qstats;
+    DeviceBuffer<float> approx;
+    DeviceBuffer<unsigned int> counters;  // [0] escalations, [1..4] MixedStats
+    DeviceBuffer<unsigned int> escalate_list;
+    DeviceBuffer<float> esc_queries;
+    DeviceBuffer<uint64_t> esc_idx;
+    DeviceBuffer<float> esc_dist;
+    void reserve_mixed(const Scan& scan);
+
+    // Optional stage timing (HIP events on the launch stream):
+    // [0] bf16 conversion, [1] Gram GEMM, [2] select + re-rank, [3] escalation.
+    static constexpr int kStages = 4;
+    bool timing = false;
+    hipEvent_t events[kStages + 1] = {};
+    float stage_ms[kStages] = {};
+    void mark(int i, hipStream_t s);
 };
 
 kern::ScanTile choose_tile(uint64_t nq, bool aligned4);
 
-// fill keys -> direct scan -> finalize, all on `stream`. forced_tile (or
-// null) overrides the tile choice.
+// fill keys -> direct scan -> finalize, all on `stream`. queries_padded
+// says every query row is readable up to the next multiple of 4 floats past
+// dim (workspace-staged rows are). forced_tile (or null) overrides the tile.
 void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t count,
                  unsigned long long* keys, uint64_t* out_idx, float* out_dist, hipStream_t stream,
-                 const kern::ScanTile* forced_tile);
+                 bool queries_padded, const kern::ScanTile* forced_tile);
+
+// Mixed-precision L2 query (BF16 Gram -> candidate set -> exact re-rank ->
+// binary64 near ties -> direct scan for escalated queries). Synchronises
+// `stream` once to read the escalation count.
+void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q,
+                  uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream);
 
 }  // namespace uhann

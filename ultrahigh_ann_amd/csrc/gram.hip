@@ -1,0 +1,440 @@
+// Mixed-precision exact L2: BF16 Gram GEMM on MFMA, a bounded candidate set
+// per query, and an exact re-rank.
+//
+// 1. to_bf16_rows: binary32 rows -> bf16 rows (round to nearest even), K
+//    padded with zeros to a multiple of 64, plus per-row statistics:
+//    ||x||^2 (binary64 sum, stored as binary32), ||x||_4, ||x^||_4, ||x^||_2.
+// 2. gram_bf16: approx[q][r] = ||q||^2 + ||v||^2 - 2 <q^, v^> with the inner
+//    product on v_mfma_f32_32x32x16_bf16 (FP32 accumulation). Block tile
+//    128 x 128, K step 64, 4 waves of 64 x 64; operands staged in LDS with a
+//    register prefetch of the next K step.
+// 3. select_rerank (one block per query):
+//    * per-pair bound tau(q,r) on |approx - exact|: bf16 rounding of both
+//      operands is a sum of independent zero-mean terms |c_i| <= u_b |q_i v_i|
+//      (u_b = 2^-8), so by Hoeffding |<q,v> - <q^,v^>| <= K u_b ||q||_4
+//      sqrt(||v||_4^2 + ||v^||_4^2) except with probability 2 exp(-K^2/2)
+//      (K = 7: 4.6e-11 per pair); the FP32 accumulation over ceil(d/16) MFMA
+//      steps adds K u_32 sqrt(ceil(d/16)) ||q^||_2 ||v^||_2, and the norms and
+//      the three-term epilogue add 4 u_32 (||q||^2 + ||v||^2);
+//    * U = min_r (approx + tau); candidates = {r : approx - tau <= U}. The
+//      exact argmin is a candidate, and no non-candidate can tie it;
+//    * the candidates are re-ranked with the reference's binary32 order
+//      (256 chains + fixed tree, as in scan.hip), so the result is the plain
+//      path's result, bit for bit;
+//    * when several candidates lie within the binary32 error band of the
+//      best (gamma_k, k = ceil(d/256) + 10), they are re-ranked once more in
+//      binary64 and the binary64 minimum (lowest row on ties) wins;
+//    * more than kMaxCandidates candidates -> the query is listed for an
+//      exact direct scan (escalation).
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "gram.hpp"
+
+namespace uhann::kern {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBM = 128;
+constexpr int kBN = 128;
+constexpr int kBK = 64;
+constexpr int kPad = 8;                 // bf16 elements of row padding in LDS
+constexpr int kLdsRow = kBK + kPad;     // 72 bf16 = 144 bytes
+constexpr int kGemmThreads = 256;
+constexpr float kHoeffdingK = 7.0F;
+constexpr float kU32 = 5.9604645e-08F;  // 2^-24
+constexpr float kUbf16 = 0.00390625F;   // 2^-8
+
+void check_launch(const char* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        throw std::runtime_error(std::string("cannot launch ") + what + ": " +
+                                 hipGetErrorString(e));
+    }
+}
+
+__device__ __forceinline__ double block_sum_f64(double v, double* scratch)
+{
+    for (int off = 32; off >= 1; off /= 2) v += __shfl_down(v, off);
+    const int lane = threadIdx.x % 64;
+    const int wave = threadIdx.x / 64;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double total = 0.0;
+    const int waves = blockDim.x / 64;
+    for (int w = 0; w < waves; ++w) total += scratch[w];
+    return total;
+}
+
+// One block of 256 threads per row.
+__global__ void __launch_bounds__(256)
+to_bf16_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows, uint32_t dim,
+                    __bf16* __restrict__ dst, uint64_t ld_dst, RowStats* __restrict__ stats)
+{
+    __shared__ double scratch[8];
+    const uint32_t row = blockIdx.x;
+    if (row >= rows) return;
+    const float* s = src + static_cast<uint64_t>(row) * ld_src;
+    __bf16* o = dst + static_cast<uint64_t>(row) * ld_dst;
+    double n2 = 0.0, n4 = 0.0, h2 = 0.0, h4 = 0.0;
+    for (uint32_t c = threadIdx.x; c < ld_dst; c += blockDim.x) {
+        const float x = c < dim ? s[c] : 0.0F;
+        const __bf16 h = static_cast<__bf16>(x);
+        o[c] = h;
+        const double xd = x;
+        const double hd = static_cast<float>(h);
+        n2 += xd * xd;
+        n4 += (xd * xd) * (xd * xd);
+        h2 += hd * hd;
+        h4 += (hd * hd) * (hd * hd);
+    }
+    n2 = block_sum_f64(n2, scratch);
+    n4 = block_sum_f64(n4, scratch);
+    h2 = block_sum_f64(h2, scratch);
+    h4 = block_sum_f64(h4, scratch);
+    if (threadIdx.x == 0) {
+        RowStats st;
+        st.norm2 = static_cast<float>(n2);
+        st.l4 = static_cast<float>(sqrt(sqrt(n4)));
+        st.l4_hat = static_cast<float>(sqrt(sqrt(h4)));
+        st.l2_hat = static_cast<float>(sqrt(h2));
+        stats[row] = st;
+    }
+}
+
+__device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nblocks)
+{
+    const uint32_t per = nblocks / 8U;
+    const uint32_t rem = nblocks % 8U;
+    const uint32_t x = b % 8U;
+    return x * per + (x < rem ? x : rem) + b / 8U;
+}
+
+// approx[m][n] = nq[m] + nv[n] - 2 <Q[m], V[n]> for m < M, n < N. Q is M x K
+// and V is N x K bf16, row-major with leading dimensions ldq / ldv, K a
+// multiple of kBK (zero padded).
+__global__ void __launch_bounds__(kGemmThreads)
+gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
+                 const __bf16* __restrict__ V, uint64_t ldv, uint32_t N, uint32_t K,
+                 const RowStats* __restrict__ qs, const RowStats* __restrict__ vs,
+                 float* __restrict__ out, uint64_t ldo, uint32_t tiles_m, uint32_t tiles_n,
+                 uint32_t group_m)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 As[kBM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN * kLdsRow];
+
+    const uint32_t L = xcd_logical(blockIdx.x, tiles_m * tiles_n);
+    const uint32_t span = group_m * tiles_n;
+    const uint32_t gid = L / span;
+    const uint32_t first_m = gid * group_m;
+    const uint32_t gsize = min(group_m, tiles_m - first_m);
+    const uint32_t within = L - gid * span;
+    const uint32_t m0 = (first_m + within % gsize) * kBM;
+    const uint32_t n0 = (within / gsize) * kBN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid % 64;
+    const int wave = tid / 64;
+    const int wm = (wave / 2) * 64;  // wave's row offset inside the block tile
+    const int wn = (wave % 2) * 64;
+
+    // Global -> register staging: each tile is 128 rows x 64 bf16 = 1024
+    // chunks of 16 bytes; thread tid moves chunks tid + 256*c.
+    uint4 ra[4], rb[4];
+    const __bf16* qrow[4];
+    const __bf16* vrow[4];
+    int lds_off[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int idx = tid + kGemmThreads * c;
+        const int row = idx / 8;
+        const int col = (idx % 8) * 8;
+        qrow[c] = Q + static_cast<uint64_t>(min(m0 + row, M - 1U)) * ldq + col;
+        vrow[c] = V + static_cast<uint64_t>(min(n0 + row, N - 1U)) * ldv + col;
+        lds_off[c] = row * kLdsRow + col;
+    }
+    auto load_global = [&](uint32_t k0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            ra[c] = *reinterpret_cast<const uint4*>(qrow[c] + k0);
+            rb[c] = *reinterpret_cast<const uint4*>(vrow[c] + k0);
+        }
+    };
+    auto store_lds = [&]() {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            *reinterpret_cast<uint4*>(As + lds_off[c]) = ra[c];
+            *reinterpret_cast<uint4*>(Bs + lds_off[c]) = rb[c];
+        }
+    };
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
+
+    const int frag_row = lane % 32;
+    const int frag_k = (lane / 32) * 8;
+
+    load_global(0);
+    store_lds();
+    __syncthreads();
+    for (uint32_t k0 = 0; k0 < K; k0 += kBK) {
+        const bool more = k0 + kBK < K;
+        if (more) load_global(k0 + kBK);
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 32 + frag_row) * kLdsRow +
+                                                        kk + frag_k);
+                b[i] = *reinterpret_cast<const bf16x8*>(Bs + (wn + i * 32 + frag_row) * kLdsRow +
+                                                        kk + frag_k);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            store_lds();
+            __syncthreads();
+        }
+    }
+
+    // D layout of the 32x32 MFMA: lane holds column (lane % 32) and rows
+    // 8*(v/4) + 4*(lane/32) + v%4 for its 16 values v.
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t n = n0 + wn + j * 32 + (lane % 32);
+            if (n >= N) continue;
+            const float nv = vs[n].norm2;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t m = m0 + wm + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (m < M) {
+                    const float base = qs[m].norm2 + nv;
+                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * acc[i][j][v];
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float pair_tau(const RowStats& q, const RowStats& v, float acc_steps_sqrt)
+{
+    const float sv = sqrtf(v.l4 * v.l4 + v.l4_hat * v.l4_hat);
+    const float dot_bound = kHoeffdingK * (kUbf16 * q.l4 * sv + kU32 * acc_steps_sqrt * q.l2_hat * v.l2_hat);
+    return 2.0F * dot_bound + 4.0F * kU32 * (q.norm2 + v.norm2);
+}
+
+// Order-preserving map of binary32 to uint32 (negative values included).
+__device__ __forceinline__ uint32_t ordered(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000U) ? ~b : (b | 0x80000000U);
+}
+__device__ __forceinline__ float unordered(uint32_t u)
+{
+    return __uint_as_float((u & 0x80000000U) ? (u & 0x7fffffffU) : ~u);
+}
+
+constexpr int kSelThreads = 256;
+constexpr int kRerankRows = 8;
+
+// One block per query. See the file comment for the algorithm.
+__global__ void __launch_bounds__(kSelThreads)
+select_rerank_kernel(SelectArgs a)
+{
+    __shared__ uint32_t s_min;
+    __shared__ uint32_t s_count;
+    __shared__ uint32_t s_cand[kMaxCandidates];
+    __shared__ float s_dist[kMaxCandidates];
+    __shared__ float s_group[8][kRerankRows];
+    __shared__ double s_scratch[8];
+    __shared__ uint32_t s_best;
+    __shared__ float s_best_d;
+
+    const uint32_t q = blockIdx.x;
+    if (q >= a.nq) return;
+    const int t = threadIdx.x;
+    const RowStats qs = a.qstats[q];
+    const float* arow = a.approx + static_cast<uint64_t>(q) * a.ld_approx;
+
+    if (t == 0) {
+        s_min = 0xffffffffU;
+        s_count = 0;
+    }
+    __syncthreads();
+    // U = min_r (approx + tau)
+    uint32_t local = 0xffffffffU;
+    for (uint32_t r = t; r < a.rows; r += kSelThreads) {
+        const float up = arow[r] + pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
+        local = min(local, ordered(up));
+    }
+    for (int off = 32; off >= 1; off /= 2) local = min(local, __shfl_down(local, off));
+    if (t % 64 == 0) atomicMin(&s_min, local);
+    __syncthreads();
+    const float upper = unordered(s_min);
+    for (uint32_t r = t; r < a.rows; r += kSelThreads) {
+        const float lo = arow[r] - pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
+        if (lo <= upper) {
+            const uint32_t slot = atomicAdd(&s_count, 1U);
+            if (slot < kMaxCandidates) s_cand[slot] = r;
+        }
+    }
+    __syncthreads();
+    const uint32_t count = s_count;
+    // No candidate only happens with non-finite estimates (overflowing
+    // norms); both cases fall back to the exact scan.
+    if (count > kMaxCandidates || count == 0) {
+        if (t == 0) {
+            const uint32_t slot = atomicAdd(a.escalate_count, 1U);
+            a.escalate_list[slot] = q;
+            a.out_idx[q] = 0;
+            a.out_dist[q] = 0.0F;
+            if (a.stats != nullptr) atomicAdd(&a.stats->escalated, 1U);
+        }
+        return;
+    }
+
+    // Exact binary32 re-rank in the reference's order: thread t is chain t.
+    const float* qrow = a.queries + static_cast<uint64_t>(q) * a.ld_q;
+    for (uint32_t g = 0; g < count; g += kRerankRows) {
+        const uint32_t ng = min(static_cast<uint32_t>(kRerankRows), count - g);
+        const float* rrow[kRerankRows];
+#pragma unroll
+        for (int j = 0; j < kRerankRows; ++j) {
+            const uint32_t r = s_cand[g + min(static_cast<uint32_t>(j), ng - 1U)];
+            rrow[j] = a.refs + static_cast<uint64_t>(r) * a.ld_ref;
+        }
+        float acc[kRerankRows];
+#pragma unroll
+        for (int j = 0; j < kRerankRows; ++j) acc[j] = 0.0F;
+        for (uint32_t c = t; c < a.dim; c += kSelThreads) {
+            const float qv = qrow[c];
+#pragma unroll
+            for (int j = 0; j < kRerankRows; ++j) {
+                const float d = rrow[j][c] - qv;
+                acc[j] = __builtin_fmaf(d, d, acc[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kRerankRows; ++j) {
+            float v = acc[j];
+            for (int off = 16; off >= 1; off /= 2) v = v + __shfl_down(v, off);
+            if (t % 32 == 0) s_group[t / 32][j] = v;
+        }
+        __syncthreads();
+        if (t < static_cast<int>(ng)) {
+            const float g0 = s_group[0][t], g1 = s_group[1][t], g2 = s_group[2][t], g3 = s_group[3][t];
+            const float g4 = s_group[4][t], g5 = s_group[5][t], g6 = s_group[6][t], g7 = s_group[7][t];
+            s_dist[g + t] = ((g0 + g4) + (g2 + g6)) + ((g1 + g5) + (g3 + g7));
+        }
+        __syncthreads();
+    }
+
+    if (t == 0) {
+        uint32_t best = 0xffffffffU;
+        float best_d = __uint_as_float(0x7f800000U);
+        for (uint32_t i = 0; i < count; ++i) {
+            const float d = s_dist[i];
+            const uint32_t r = s_cand[i];
+            if (d < best_d || (d == best_d && r < best)) {
+                best_d = d;
+                best = r;
+            }
+        }
+        s_best = best;
+        s_best_d = best_d;
+    }
+    __syncthreads();
+    uint32_t best = s_best;
+    float best_d = s_best_d;
+
+    // Binary64 re-rank of candidates inside the binary32 error band.
+    const double band = static_cast<double>(best_d) * a.fp32_band;
+    uint32_t near = 0;
+    for (uint32_t i = 0; i < count; ++i) near += (static_cast<double>(s_dist[i]) <= band) ? 1U : 0U;
+    if (near >= 2) {
+        double best64 = __longlong_as_double(0x7ff0000000000000LL);
+        uint32_t best64_r = 0xffffffffU;
+        float best64_f = 0.0F;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (static_cast<double>(s_dist[i]) > band) continue;
+            const uint32_t r = s_cand[i];
+            const float* rr = a.refs + static_cast<uint64_t>(r) * a.ld_ref;
+            double part = 0.0;
+            for (uint32_t c = t; c < a.dim; c += kSelThreads) {
+                const double d = static_cast<double>(rr[c]) - static_cast<double>(qrow[c]);
+                part += d * d;
+            }
+            const double d64 = block_sum_f64(part, s_scratch);
+            if (d64 < best64 || (d64 == best64 && r < best64_r)) {
+                best64 = d64;
+                best64_r = r;
+                best64_f = s_dist[i];
+            }
+        }
+        best = best64_r;
+        best_d = best64_f;
+        if (t == 0 && a.stats != nullptr) atomicAdd(&a.stats->fp64_reranked, 1U);
+    }
+    if (t == 0) {
+        a.out_idx[q] = best;
+        a.out_dist[q] = best_d;
+        if (a.stats != nullptr) atomicAdd(&a.stats->candidates, count);
+    }
+}
+
+}  // namespace
+
+void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, __bf16* dst,
+                  uint64_t ld_dst, RowStats* stats, hipStream_t stream)
+{
+    if (rows == 0) return;
+    to_bf16_rows_kernel<<<rows, 256, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    check_launch("bf16 conversion kernel");
+}
+
+void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint64_t ldv, uint32_t n,
+               uint32_t k, const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo,
+               hipStream_t stream)
+{
+    if (m == 0 || n == 0) return;
+    if (k % kBK != 0) throw std::invalid_argument("Gram K must be a multiple of 64");
+    const uint32_t tiles_m = (m + kBM - 1) / kBM;
+    const uint32_t tiles_n = (n + kBN - 1) / kBN;
+    const uint32_t group_m = 8;
+    gram_bf16_kernel<<<tiles_m * tiles_n, kGemmThreads, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs,
+                                                                    out, ldo, tiles_m, tiles_n,
+                                                                    group_m);
+    check_launch("bf16 Gram kernel");
+}
+
+void select_rerank(const SelectArgs& args, hipStream_t stream)
+{
+    if (args.nq == 0) return;
+    select_rerank_kernel<<<args.nq, kSelThreads, 0, stream>>>(args);
+    check_launch("candidate selection / re-rank kernel");
+}
+
+uint32_t gram_k_padding() { return kBK; }
+uint32_t gram_m_tile() { return kBM; }
+
+}  // namespace uhann::kern

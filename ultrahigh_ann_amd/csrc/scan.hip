@@ -126,7 +126,11 @@ scan_direct_kernel(ScanArgs a, uint32_t tiles_q, uint32_t tiles_r, uint32_t grou
 #pragma unroll
             for (int k = 0; k < RPT; ++k) acc[i][j][k] = 0.0F;
 
-    auto step = [&](uint32_t col) {
+    // One 256-column step at column `col` (this thread's first chain). In the
+    // tail step a vector may straddle dim (rows are padded to a multiple of 4
+    // floats): components at or past dim are skipped, so pad values are
+    // never added.
+    auto step = [&](uint32_t col, bool masked) {
         float rv[TR][RPT];
         float qv[TQ][RPT];
         float wv[RPT];
@@ -141,20 +145,22 @@ scan_direct_kernel(ScanArgs a, uint32_t tiles_q, uint32_t tiles_r, uint32_t grou
             for (int k = 0; k < RPT; ++k) wv[k] = 0.0F;
         }
 #pragma unroll
-        for (int i = 0; i < TQ; ++i)
+        for (int k = 0; k < RPT; ++k) {
+            if (masked && col + static_cast<uint32_t>(k) >= a.dim) continue;
 #pragma unroll
-            for (int j = 0; j < TR; ++j)
+            for (int i = 0; i < TQ; ++i)
 #pragma unroll
-                for (int k = 0; k < RPT; ++k)
+                for (int j = 0; j < TR; ++j)
                     acc[i][j][k] = accumulate<METRIC, WEIGHTED, WFMA>(acc[i][j][k], rv[j][k],
                                                                      qv[i][k], wv[k]);
+        }
     };
 
     const uint32_t col_t = static_cast<uint32_t>(RPT * t);
     const uint32_t full_end = (a.dim / kChains) * kChains;
 #pragma unroll 2
-    for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t);
-    if (full_end + col_t < a.dim) step(full_end + col_t);
+    for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false);
+    if (full_end + col_t < a.dim) step(full_end + col_t, true);
 
     // Replay the reference tree. Cross-thread steps first (chain offsets 16..RPT
     // are thread offsets GROUP/2..1), then the chains inside the thread.
