@@ -10,6 +10,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -204,6 +205,14 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
 }
 
 
+// Gram block shape; UHANN_GRAM_TILE (0..3) overrides the default for tuning.
+int gram_tile()
+{
+    const char* env = std::getenv("UHANN_GRAM_TILE");
+    if (env != nullptr && env[0] >= '0' && env[0] <= '3' && env[1] == '\0') return env[0] - '0';
+    return 3;
+}
+
 void Scan::enable_mixed()
 {
     if (metric != UHANN_METRIC_L2) {
@@ -258,7 +267,7 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
     ws.mark(1, stream);
     kern::gram_bf16(ws.q16.data(), g.ldk, n, g.refs16.data(), g.ldk, static_cast<uint32_t>(scan.rows),
                     static_cast<uint32_t>(g.ldk), ws.qstats.data(), g.stats.data(), ws.approx.data(),
-                    scan.rows, stream);
+                    scan.rows, gram_tile(), stream);
     ws.mark(2, stream);
     kern::zero_u32(ws.counters.data(), 8, stream);
     kern::SelectArgs a;

@@ -5,9 +5,10 @@
 //    padded with zeros to a multiple of 64, plus per-row statistics:
 //    ||x||^2 (binary64 sum, stored as binary32), ||x||_4, ||x^||_4, ||x^||_2.
 // 2. gram_bf16: approx[q][r] = ||q||^2 + ||v||^2 - 2 <q^, v^> with the inner
-//    product on v_mfma_f32_32x32x16_bf16 (FP32 accumulation). Block tile
-//    128 x 128, K step 64, 4 waves of 64 x 64; operands staged in LDS with a
-//    register prefetch of the next K step.
+//    product on v_mfma_f32_32x32x16_bf16 (FP32 accumulation). Blocks are a
+//    grid of 64 x 64 wave tiles (128 x 128 up to 256 x 256), K step 64,
+//    operands double-buffered in LDS with a register prefetch of the next K
+//    step.
 // 3. select_rerank (one block per query):
 //    * per-pair bound tau(q,r) on |approx - exact|: bf16 rounding of both
 //      operands is a sum of independent zero-mean terms |c_i| <= u_b |q_i v_i|
@@ -40,12 +41,9 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int kBM = 128;
-constexpr int kBN = 128;
 constexpr int kBK = 64;
 constexpr int kPad = 8;                 // bf16 elements of row padding in LDS
 constexpr int kLdsRow = kBK + kPad;     // 72 bf16 = 144 bytes
-constexpr int kGemmThreads = 256;
 constexpr float kHoeffdingK = 7.0F;
 constexpr float kU32 = 5.9604645e-08F;  // 2^-24
 constexpr float kUbf16 = 0.00390625F;   // 2^-8
@@ -117,18 +115,56 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nblocks)
     return x * per + (x < rem ? x : rem) + b / 8U;
 }
 
+// Global -> register staging of one K step: an operand tile is ROWS x 64
+// bf16 = ROWS * 8 chunks of 16 bytes; thread tid moves chunks tid + NT*c.
+#define UHANN_GRAM_LOAD(k0)                                                         \
+    _Pragma("unroll") for (int c = 0; c < CA; ++c) ra[c] =                          \
+        *reinterpret_cast<const uint4*>(qrow[c] + (k0));                            \
+    _Pragma("unroll") for (int c = 0; c < CB; ++c) rb[c] =                          \
+        *reinterpret_cast<const uint4*>(vrow[c] + (k0));                            \
+    __builtin_amdgcn_sched_barrier(0);
+#define UHANN_GRAM_STORE(buf)                                                       \
+    _Pragma("unroll") for (int c = 0; c < CA; ++c)                                  \
+        *reinterpret_cast<uint4*>(As + (buf) * BM * kLdsRow + a_lds[c]) = ra[c];    \
+    _Pragma("unroll") for (int c = 0; c < CB; ++c)                                  \
+        *reinterpret_cast<uint4*>(Bs + (buf) * BN * kLdsRow + b_lds[c]) = rb[c];
+// Four K=16 MFMA steps over one LDS stage for the wave's 2 x 2 32x32 tiles.
+#define UHANN_GRAM_COMPUTE(buf)                                                     \
+    _Pragma("unroll") for (int kk = 0; kk < kBK; kk += 16)                          \
+    {                                                                               \
+        const __bf16* ab = As + (buf) * BM * kLdsRow + a_off + kk;                  \
+        const __bf16* bb = Bs + (buf) * BN * kLdsRow + b_off + kk;                  \
+        bf16x8 a0 = *reinterpret_cast<const bf16x8*>(ab);                           \
+        bf16x8 a1 = *reinterpret_cast<const bf16x8*>(ab + 32 * kLdsRow);            \
+        bf16x8 b0 = *reinterpret_cast<const bf16x8*>(bb);                           \
+        bf16x8 b1 = *reinterpret_cast<const bf16x8*>(bb + 32 * kLdsRow);            \
+        acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc00, 0, 0, 0);    \
+        acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc01, 0, 0, 0);    \
+        acc10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc10, 0, 0, 0);    \
+        acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc11, 0, 0, 0);    \
+    }
+
 // approx[m][n] = nq[m] + nv[n] - 2 <Q[m], V[n]> for m < M, n < N. Q is M x K
 // and V is N x K bf16, row-major with leading dimensions ldq / ldv, K a
-// multiple of kBK (zero padded).
-__global__ void __launch_bounds__(kGemmThreads)
+// multiple of kBK (zero padded). A block is a WM x WN grid of waves, each
+// computing a 64 x 64 tile with 2 x 2 v_mfma_f32_32x32x16_bf16. Two LDS
+// stages: while the MFMAs consume one, the next K step travels global ->
+// registers -> the other, so each K step needs a single barrier.
+template <int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN)
 gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
                  const __bf16* __restrict__ V, uint64_t ldv, uint32_t N, uint32_t K,
                  const RowStats* __restrict__ qs, const RowStats* __restrict__ vs,
                  float* __restrict__ out, uint64_t ldo, uint32_t tiles_m, uint32_t tiles_n,
                  uint32_t group_m)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 As[kBM * kLdsRow];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[kBN * kLdsRow];
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BM = 64 * WM;
+    constexpr int BN = 64 * WN;
+    constexpr int CA = BM * 8 / NT;  // 16-byte chunks per thread per K step
+    constexpr int CB = BN * 8 / NT;
+    __shared__ __attribute__((aligned(16))) __bf16 As[2 * BM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * BN * kLdsRow];
 
     const uint32_t L = xcd_logical(blockIdx.x, tiles_m * tiles_n);
     const uint32_t span = group_m * tiles_n;
@@ -136,87 +172,61 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
     const uint32_t first_m = gid * group_m;
     const uint32_t gsize = min(group_m, tiles_m - first_m);
     const uint32_t within = L - gid * span;
-    const uint32_t m0 = (first_m + within % gsize) * kBM;
-    const uint32_t n0 = (within / gsize) * kBN;
+    const uint32_t m0 = (first_m + within % gsize) * BM;
+    const uint32_t n0 = (within / gsize) * BN;
 
     const int tid = threadIdx.x;
     const int lane = tid % 64;
     const int wave = tid / 64;
-    const int wm = (wave / 2) * 64;  // wave's row offset inside the block tile
-    const int wn = (wave % 2) * 64;
+    const int wm = (wave / WN) * 64;  // wave's offsets inside the block tile
+    const int wn = (wave % WN) * 64;
 
-    // Global -> register staging: each tile is 128 rows x 64 bf16 = 1024
-    // chunks of 16 bytes; thread tid moves chunks tid + 256*c.
-    uint4 ra[4], rb[4];
-    const __bf16* qrow[4];
-    const __bf16* vrow[4];
-    int lds_off[4];
+    uint4 ra[CA], rb[CB];
+    const __bf16* qrow[CA];
+    const __bf16* vrow[CB];
+    int a_lds[CA], b_lds[CB];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int idx = tid + kGemmThreads * c;
-        const int row = idx / 8;
-        const int col = (idx % 8) * 8;
-        qrow[c] = Q + static_cast<uint64_t>(min(m0 + row, M - 1U)) * ldq + col;
-        vrow[c] = V + static_cast<uint64_t>(min(n0 + row, N - 1U)) * ldv + col;
-        lds_off[c] = row * kLdsRow + col;
+    for (int c = 0; c < CA; ++c) {
+        const int idx = tid + NT * c;
+        qrow[c] = Q + static_cast<uint64_t>(min(m0 + idx / 8, M - 1U)) * ldq + (idx % 8) * 8;
+        a_lds[c] = (idx / 8) * kLdsRow + (idx % 8) * 8;
     }
-    auto load_global = [&](uint32_t k0) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            ra[c] = *reinterpret_cast<const uint4*>(qrow[c] + k0);
-            rb[c] = *reinterpret_cast<const uint4*>(vrow[c] + k0);
-        }
-    };
-    auto store_lds = [&]() {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            *reinterpret_cast<uint4*>(As + lds_off[c]) = ra[c];
-            *reinterpret_cast<uint4*>(Bs + lds_off[c]) = rb[c];
-        }
-    };
+    for (int c = 0; c < CB; ++c) {
+        const int idx = tid + NT * c;
+        vrow[c] = V + static_cast<uint64_t>(min(n0 + idx / 8, N - 1U)) * ldv + (idx % 8) * 8;
+        b_lds[c] = (idx / 8) * kLdsRow + (idx % 8) * 8;
+    }
+    // MFMA operand fragments: lane holds row (lane % 32), K values
+    // 8*(lane/32) .. +7 of each 16-wide K step.
+    const int a_off = (wm + lane % 32) * kLdsRow + (lane / 32) * 8;
+    const int b_off = (wn + lane % 32) * kLdsRow + (lane / 32) * 8;
 
-    floatx16 acc[2][2];
+    floatx16 acc00, acc01, acc10, acc11;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
+    for (int v = 0; v < 16; ++v) {
+        acc00[v] = 0.0F;
+        acc01[v] = 0.0F;
+        acc10[v] = 0.0F;
+        acc11[v] = 0.0F;
+    }
 
-    const int frag_row = lane % 32;
-    const int frag_k = (lane / 32) * 8;
-
-    load_global(0);
-    store_lds();
+    UHANN_GRAM_LOAD(0)
+    UHANN_GRAM_STORE(0)
     __syncthreads();
-    for (uint32_t k0 = 0; k0 < K; k0 += kBK) {
-        const bool more = k0 + kBK < K;
-        if (more) load_global(k0 + kBK);
-#pragma unroll
-        for (int kk = 0; kk < kBK; kk += 16) {
-            bf16x8 a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = *reinterpret_cast<const bf16x8*>(As + (wm + i * 32 + frag_row) * kLdsRow +
-                                                        kk + frag_k);
-                b[i] = *reinterpret_cast<const bf16x8*>(Bs + (wn + i * 32 + frag_row) * kLdsRow +
-                                                        kk + frag_k);
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
+    int cur = 0;
+    for (uint32_t k0 = kBK; k0 < K; k0 += kBK) {
+        UHANN_GRAM_LOAD(k0)
+        UHANN_GRAM_COMPUTE(cur)
+        UHANN_GRAM_STORE(cur ^ 1)
         __syncthreads();
-        if (more) {
-            store_lds();
-            __syncthreads();
-        }
+        cur ^= 1;
     }
+    UHANN_GRAM_COMPUTE(cur)
 
     // D layout of the 32x32 MFMA: lane holds column (lane % 32) and rows
     // 8*(v/4) + 4*(lane/32) + v%4 for its 16 values v.
+    const floatx16* accs[2][2] = {{&acc00, &acc01}, {&acc10, &acc11}};
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
@@ -229,12 +239,16 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
                 const uint32_t m = m0 + wm + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
                 if (m < M) {
                     const float base = qs[m].norm2 + nv;
-                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * acc[i][j][v];
+                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * (*accs[i][j])[v];
                 }
             }
         }
     }
 }
+
+#undef UHANN_GRAM_LOAD
+#undef UHANN_GRAM_STORE
+#undef UHANN_GRAM_COMPUTE
 
 __device__ __forceinline__ float pair_tau(const RowStats& q, const RowStats& v, float acc_steps_sqrt)
 {
@@ -414,16 +428,39 @@ void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim
 
 void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint64_t ldv, uint32_t n,
                uint32_t k, const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo,
-               hipStream_t stream)
+               int tile, hipStream_t stream)
 {
     if (m == 0 || n == 0) return;
     if (k % kBK != 0) throw std::invalid_argument("Gram K must be a multiple of 64");
-    const uint32_t tiles_m = (m + kBM - 1) / kBM;
-    const uint32_t tiles_n = (n + kBN - 1) / kBN;
+    // tile: 0 -> 2x2 waves (128x128), 1 -> 4x2 (256x128), 2 -> 2x4 (128x256),
+    // 3 -> 4x4 (256x256).
+    static const int wm_of[4] = {2, 4, 2, 4};
+    static const int wn_of[4] = {2, 2, 4, 4};
+    if (tile < 0 || tile > 3) throw std::invalid_argument("unknown Gram tile");
+    const uint32_t bm = 64U * wm_of[tile];
+    const uint32_t bn = 64U * wn_of[tile];
+    const uint32_t tiles_m = (m + bm - 1) / bm;
+    const uint32_t tiles_n = (n + bn - 1) / bn;
     const uint32_t group_m = 8;
-    gram_bf16_kernel<<<tiles_m * tiles_n, kGemmThreads, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs,
-                                                                    out, ldo, tiles_m, tiles_n,
-                                                                    group_m);
+    const dim3 grid(tiles_m * tiles_n);
+    switch (tile) {
+    case 0:
+        gram_bf16_kernel<2, 2><<<grid, 256, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
+                                                        tiles_m, tiles_n, group_m);
+        break;
+    case 1:
+        gram_bf16_kernel<4, 2><<<grid, 512, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
+                                                        tiles_m, tiles_n, group_m);
+        break;
+    case 2:
+        gram_bf16_kernel<2, 4><<<grid, 512, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
+                                                        tiles_m, tiles_n, group_m);
+        break;
+    default:
+        gram_bf16_kernel<4, 4><<<grid, 1024, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
+                                                         tiles_m, tiles_n, group_m);
+        break;
+    }
     check_launch("bf16 Gram kernel");
 }
 
@@ -435,6 +472,6 @@ void select_rerank(const SelectArgs& args, hipStream_t stream)
 }
 
 uint32_t gram_k_padding() { return kBK; }
-uint32_t gram_m_tile() { return kBM; }
+uint32_t gram_m_tile() { return 256; }
 
 }  // namespace uhann::kern

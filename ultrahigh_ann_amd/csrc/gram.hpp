@@ -48,9 +48,11 @@ struct SelectArgs {
 
 void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, __bf16* dst,
                   uint64_t ld_dst, RowStats* stats, hipStream_t stream);
+// tile selects the block shape (0: 128x128, 1: 256x128, 2: 128x256,
+// 3: 256x256 outputs).
 void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint64_t ldv, uint32_t n,
                uint32_t k, const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo,
-               hipStream_t stream);
+               int tile, hipStream_t stream);
 void select_rerank(const SelectArgs& args, hipStream_t stream);
 uint32_t gram_k_padding();
 uint32_t gram_m_tile();
