@@ -152,6 +152,128 @@ int uhann_workspace_mixed_stats(uhann_workspace* ws, uint64_t* candidates, uint6
 int uhann_workspace_set_timing(uhann_workspace* ws, int enable);
 int uhann_workspace_stage_times(const uhann_workspace* ws, float* ms, int n);
 
+/* ==== indexes ============================================================
+ * Exact, flat (importance-sampled coordinate multiset I) and hierarchical
+ * (I projected to p dimensions) indexes, L2 or L1. Queries are full-width
+ * (dim) rows; the gather of I runs on the device. */
+typedef struct uhann_index uhann_index;
+
+/* One sampled coordinate, the reference's SampledColumn
+ * (include/ultrahigh_ann/coordinate_sampling/coordinate_sample.hpp:10-14). */
+typedef struct uhann_sampled_column {
+    uint64_t source_column;
+    uint64_t multiplicity;
+    double inverse_probability;
+} uhann_sampled_column;
+
+/* Exact index. Replaces CudaExactL2Index(const DenseMatrix&, int device)
+ * (include/ultrahigh_ann/exact/l2/cuda_exact_l2_index.hpp) and
+ * CudaExactL1Index (exact/l1/cuda_exact_l1_index.hpp). */
+int uhann_exact_create(int metric, const float* host_refs, uint64_t rows, uint64_t dim, int device,
+                       uhann_index** out);
+
+/* Flat index from an explicit sample (ascending source columns). L2 scales
+ * the sampled references by (float)sqrt(m/p); L1 keeps them raw and weighs
+ * |r - q| by (float)(m/p). An empty sample scans zero coordinates (row 0).
+ * Replaces CudaSampledCoordinateL2AnnIndex(const DenseMatrix&,
+ * CoordinateSample, int device) (coordinate_sampling/
+ * cuda_sampled_coordinate_l2_ann_index.hpp:45-48) and its L1 twin
+ * (cuda_sampled_coordinate_l1_ann_index.hpp); the probability-based
+ * CudaFlatL{1,2}AnnIndex constructors (hnsvw25/l{1,2}/flat/cuda_l*_ann_index.hpp)
+ * are uhann_build_coordinate_sample + this call. Errors: column out of range
+ * -> OUT_OF_RANGE; zero multiplicity or bad weights -> INVALID_ARGUMENT. */
+int uhann_flat_create(int metric, const float* host_refs, uint64_t rows, uint64_t dim,
+                      const uhann_sampled_column* sample, uint64_t sample_count, int device,
+                      uhann_index** out);
+
+/* Hierarchical index from an explicit sample and its binary64 projection
+ * matrix (projection_dim x sample_count, row-major, as drawn by
+ * uhann_build_projection). The matrix is narrowed to binary32 and the sampled
+ * references projected on the host in binary32, as the reference's FP32
+ * build does; queries are projected on the device in the reference's K7
+ * order and scanned with squared L2 (L2) or the median of |differences|
+ * (L1, projection_dim <= 128). Replaces CudaHierarchicalL2AnnIndex(const
+ * DenseMatrix&, CoordinateSample, p, engine, device)
+ * (hnsvw25/l2/hierarchical/cuda_hierarchical_l2_ann_index.hpp) and
+ * CudaHierarchicalL1AnnIndex. Empty sample with rows > 1 -> RUNTIME. */
+int uhann_hierarchical_create(int metric, const float* host_refs, uint64_t rows, uint64_t dim,
+                              const uhann_sampled_column* sample, uint64_t sample_count,
+                              const double* projection, uint64_t projection_dim, int device,
+                              uhann_index** out);
+
+void uhann_index_destroy(uhann_index* index);
+
+/* Replaces space_usage()/IndexSpaceUsage (include/ultrahigh_ann/core/
+ * index_space_usage.hpp): kind, metric, shape, unique sampled coordinates,
+ * summed multiplicity and payload bytes counted the reference's way. */
+int uhann_index_info(const uhann_index* index, int* kind, int* metric, uint64_t* rows, uint64_t* dim,
+                     uint64_t* unique_coordinates, uint64_t* multiplicity, uint64_t* payload_bytes);
+
+/* Prepares GRAM_BF16 for an exact or flat L2 index (see uhann_scan_enable_mixed). */
+int uhann_index_enable_mixed(uhann_index* index, int strategy);
+
+/* Replaces make_query_workspace(max_batch) of every facade. */
+int uhann_index_workspace_create(const uhann_index* index, uint64_t max_batch,
+                                 uhann_workspace** out);
+
+/* Synchronous batch from host queries (count x dim). Replaces the facades'
+ * query_batch (e.g. cuda_sampled_coordinate_l2_ann_index.hpp:63-68). */
+int uhann_index_query_batch(const uhann_index* index, uhann_workspace* ws, const float* host_queries,
+                            uint64_t count, uint64_t* out_idx, float* out_dist, int strategy);
+
+/* Synchronous batch from device queries (row stride dim), host results. */
+int uhann_index_query_device_batch(const uhann_index* index, uhann_workspace* ws,
+                                   const float* device_queries, uint64_t count, uint64_t* out_idx,
+                                   float* out_dist, int strategy);
+
+/* Asynchronous device-to-device batch on a caller stream. Data errors (a
+ * non-finite sampled coordinate, projection or median) are kept on the
+ * workspace and reported by uhann_workspace_status. No counterpart in the
+ * reference. */
+int uhann_index_query_device_async(const uhann_index* index, uhann_workspace* ws,
+                                   const float* device_queries, uint64_t ld_queries, uint64_t count,
+                                   uint64_t* device_out_idx, float* device_out_dist, int strategy,
+                                   void* stream);
+
+/* Waits for the device and reports (then clears) a data error left by
+ * asynchronous batches: INVALID_ARGUMENT for a non-finite sampled query
+ * coordinate, OVERFLOW for a non-finite projection or median. */
+int uhann_workspace_status(uhann_workspace* ws);
+
+/* ==== host index construction (reference <random> order) =================
+ * A std::mt19937_64 held behind a handle; the draws match the reference's
+ * libstdc++ distributions call for call. */
+typedef struct uhann_rng uhann_rng;
+
+/* std::mt19937_64(seed), the engine every reference constructor takes. */
+int uhann_rng_create(uint64_t seed, uhann_rng** out);
+void uhann_rng_destroy(uhann_rng* rng);
+/* *equal = 1 when both engines are in the same state (the reference's
+ * "consume identical randomness" check, cuda_hierarchical_l2_ann_index_test.cpp:88-90). */
+int uhann_rng_equal(const uhann_rng* a, const uhann_rng* b, int* equal);
+/* Next raw 64-bit draw. */
+int uhann_rng_next(uhann_rng* rng, uint64_t* value);
+
+/* build_coordinate_sample (coordinate_sampling.hpp, coordinate_sampling.cpp:
+ * 14-45). Writes the kept columns to out (capacity entries; dim always
+ * suffices) and their number to *count. Invalid probability -> DOMAIN. */
+int uhann_build_coordinate_sample(uhann_rng* rng, const double* probabilities, uint64_t dim,
+                                  uint64_t repetitions, uhann_sampled_column* out, uint64_t capacity,
+                                  uint64_t* count);
+
+/* Projection matrix drawn after the sample: L2 JL (Rademacher sums,
+ * l2_hierarchical_projection.cpp:53-77), L1 Cauchy
+ * (l1_hierarchical_projection.cpp:54-77); out is projection_dim x count
+ * binary64, row-major. */
+int uhann_build_projection(uhann_rng* rng, int metric, const uhann_sampled_column* sample,
+                           uint64_t count, uint64_t projection_dim, double* out);
+
+/* Importance probabilities on the host in binary64 (L2:
+ * compute_l2_importance_probabilities, hnsvw25/l2/importance_sampling.cpp:
+ * 18-107; L1: hnsvw25/l1/importance_sampling.cpp:19-104), dim doubles. */
+int uhann_importance_probabilities(int metric, const float* refs, uint64_t rows, uint64_t dim,
+                                   double* out);
+
 #ifdef __cplusplus
 }
 #endif

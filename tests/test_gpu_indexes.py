@@ -1,0 +1,247 @@
+"""GPU parity of the flat and hierarchical indexes through the C ABI.
+
+Fixtures follow the reference's CUDA tests
+(tests/coordinate_sampling/cuda_sampled_coordinate_l2_ann_index_test.cpp,
+tests/hnsvw25/l{1,2}/hierarchical/cuda_hierarchical_l*_ann_index_test.cpp).
+Random cases are checked bit for bit against the oracle's emulation of the
+reference's CUDA arithmetic: flat = embedded references/queries then K2 (L2)
+or weighted K5 (L1); hierarchical = K7 projection of the gathered coordinates
+then K2 over p (L2) or the K9 median (L1)."""
+
+import numpy as np
+import pytest
+
+from conftest import rng
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+NAN, INF = np.float32(np.nan), np.float32(np.inf)
+
+REPS = np.array([[0, 99, 0, -5], [4, -99, 1, -5], [2, 0, 5, -5]], F32)
+Q4 = np.array([[0.5, 10, 0.25, 20], [3.8, -10, 1.1, 20], [2, 10, 0.5, 20], [2, -10, 5, 20]], F32)
+
+
+def _emb(x, cols, scales=None):
+    g = np.ascontiguousarray(x[:, cols.astype(np.int64)], F32)
+    return g if scales is None else (g * scales).astype(F32)
+
+
+def test_flat_l2_reference_fixture(uh, oracle_mod):
+    sample = uh.make_sample([0, 2], [2, 1], [0.5, 4.0])
+    index = uh.HipSampledCoordinateL2AnnIndex(REPS, sample)
+    ws = index.make_query_workspace(4)
+    use = index.space_usage()
+    assert use["index_payload_bytes"] == 3 * 2 * 4 + 3 * 4 + 2 * 8 + 2 * 4
+    assert use["unique_query_coordinates"] == 2 and use["sampled_multiplicity"] == 3
+    idx, dist = index.query_batch(Q4, 4, ws)
+    scales, _ = oracle_mod.embedding_factors(sample["multiplicity"], sample["inverse_probability"])
+    cols = sample["source_column"]
+    exp64, _ = oracle_mod.l2_fp64_scan(_emb(REPS, cols, scales), _emb(Q4, cols, scales))
+    np.testing.assert_array_equal(idx, exp64)
+    assert idx[2] == 0  # tie resolves to the lowest row
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(_emb(REPS, cols, scales), _emb(Q4, cols, scales)))
+    np.testing.assert_array_equal(dist.view(np.uint32), exp_dist.view(np.uint32))
+    # unsampled non-finite coordinates are never read
+    assert index.query(np.array([0.5, NAN, 0.25, INF], F32), ws) == index.query(np.array([0.5, 0, 0.25, 0], F32), ws)
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query(np.array([NAN, 0, 0, 0], F32), ws)
+    # empty sample -> row 0
+    empty = uh.HipSampledCoordinateL2AnnIndex(REPS, uh.make_sample([], [], []))
+    ews = empty.make_query_workspace(2)
+    assert list(empty.query_batch(Q4[:2], 2, ews)[0]) == [0, 0]
+    # invalid sample
+    with pytest.raises(uh.UhannInvalidArgument):
+        uh.HipSampledCoordinateL2AnnIndex(REPS, uh.make_sample([0], [0], [1.0]))
+    with pytest.raises(uh.UhannOutOfRange):
+        uh.HipSampledCoordinateL2AnnIndex(REPS, uh.make_sample([9], [1], [1.0]))
+    # foreign workspace and capacity
+    other = uh.HipSampledCoordinateL2AnnIndex(REPS, sample)
+    with pytest.raises(uh.UhannInvalidArgument):
+        other.query(Q4[0], ws)
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query_batch(np.zeros((5, 4), F32), 5, ws)
+
+
+def test_flat_seed_42_matches_cpu_flat(uh, oracle_mod, ref_mod):
+    # cuda_sampled_coordinate_l2_ann_index_test.cpp:169-186
+    probs = np.array([1.0, 0.0, 1.0, 0.0])
+    index = uh.HipFlatL2AnnIndex(REPS, probs, 2, uh.Rng(42))
+    ws = index.make_query_workspace(4)
+    idx, _ = index.query_batch(Q4, 4, ws)
+    theirs = np.zeros(4, np.uint64)
+    assert ref_mod.ref().ref_flat(0, ref_mod._p(REPS, ref_mod._f32p), 3, 4, ref_mod._p(probs, ref_mod._f64p), 2, 42,
+                                  ref_mod._p(Q4, ref_mod._f32p), 4, ref_mod._p(theirs, ref_mod._u64p)) == 0
+    np.testing.assert_array_equal(idx, theirs)
+
+
+def test_flat_l1_reference_fixture(uh, oracle_mod):
+    sample = uh.make_sample([0, 2], [2, 1], [0.5, 4.0])
+    index = uh.HipSampledCoordinateL1AnnIndex(REPS, sample)
+    ws = index.make_query_workspace(4)
+    idx, dist = index.query_batch(Q4, 4, ws)
+    cols, mult, invp = sample["source_column"], sample["multiplicity"], sample["inverse_probability"]
+    exp64, _ = oracle_mod.weighted_l1_fp64_scan(_emb(REPS, cols), Q4, cols, mult, invp)
+    np.testing.assert_array_equal(idx, exp64)
+    _, weights = oracle_mod.embedding_factors(mult, invp)
+    e_idx, e_dist = oracle_mod.fp32_argmin(oracle_mod.k5_distances(_emb(REPS, cols), _emb(Q4, cols), weights, 0))
+    np.testing.assert_array_equal(idx, e_idx)
+    np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
+
+
+def _random_index_case(seed, r, d, b):
+    g = rng(seed)
+    refs = (g.random((r, d), dtype=np.float32) * 3).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 3).astype(F32)
+    return refs, q
+
+
+@pytest.mark.parametrize("metric", ["l2", "l1"])
+@pytest.mark.parametrize("r,d,b,T", [(60, 900, 17, 16), (300, 4000, 40, 64), (7, 33, 5, 3)])
+def test_flat_parity_bit_exact(uh, oracle_mod, metric, r, d, b, T):
+    import torch
+
+    refs, q = _random_index_case(r + d, r, d, b)
+    l1 = metric == "l1"
+    probs = oracle_mod.importance_probabilities(l1, refs)
+    cls = uh.HipFlatL1AnnIndex if l1 else uh.HipFlatL2AnnIndex
+    index = cls(refs, probs, T, uh.Rng(2026))
+    ws = index.make_query_workspace(b)
+    cols, mult, invp = oracle_mod.build_coordinate_sample(probs, T, 2026)
+    np.testing.assert_array_equal(index.sample["source_column"], cols)
+    scales, weights = oracle_mod.embedding_factors(mult, invp)
+    if l1:
+        emul = oracle_mod.k5_distances(_emb(refs, cols), _emb(q, cols), weights, 0)
+    else:
+        emul = oracle_mod.k2_distances(_emb(refs, cols, scales), _emb(q, cols, scales))
+    e_idx, e_dist = oracle_mod.fp32_argmin(emul)
+    idx, dist = index.query_batch(q, b, ws)
+    np.testing.assert_array_equal(idx, e_idx)
+    np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
+    didx, ddist = index.query_device_batch(torch.from_numpy(q).cuda(), b, ws)
+    np.testing.assert_array_equal(didx, e_idx)
+    np.testing.assert_array_equal(ddist.view(np.uint32), e_dist.view(np.uint32))
+
+
+def test_flat_l2_mixed_matches_direct(uh, oracle_mod):
+    refs, q = _random_index_case(5, 400, 6000, 30)
+    probs = oracle_mod.importance_probabilities(False, refs)
+    index = uh.HipFlatL2AnnIndex(refs, probs, 128, uh.Rng(7))
+    index.enable_mixed()
+    ws = index.make_query_workspace(30)
+    d_idx, d_dist = index.query_batch(q, 30, ws)
+    m_idx, m_dist = index.query_batch(q, 30, ws, uh.STRATEGY_GRAM_BF16)
+    same = m_idx == d_idx
+    np.testing.assert_array_equal(m_dist[same].view(np.uint32), d_dist[same].view(np.uint32))
+    assert same.mean() >= 0.9
+
+
+def _hier_expect(oracle_mod, metric_l1, refs, q, probs, T, p, seed):
+    built = oracle_mod.build_hierarchical(metric_l1, refs, probs, T, p, seed)
+    x = _emb(q, built["cols"])
+    y, bad = oracle_mod.k7_project(x, built["p32"])
+    assert not bad
+    if metric_l1:
+        dist, _ = oracle_mod.k9_median_l1(built["proj_refs32"], y)
+    else:
+        dist = oracle_mod.k2_distances(built["proj_refs32"], y)
+    return built, oracle_mod.fp32_argmin(dist)
+
+
+def test_hierarchical_l2_reference_fixture(uh, oracle_mod):
+    # cuda_hierarchical_l2_ann_index_test.cpp:50-187
+    probs = np.array([1.0, 0.0, 1.0, 0.0])
+    q = np.array([[0.2, 10, 0.1, 20], [3.9, -10, 1.1, 20], [2.1, 10, 4.8, 20], [1.0, -10, 2.0, 20]], F32)
+    r = uh.Rng(1729)
+    index = uh.HipHierarchicalL2AnnIndex(REPS, probs, 1, 7, r)
+    r2 = uh.Rng(1729)
+    s2 = uh.build_coordinate_sample(probs, 1, r2)
+    uh.build_projection(s2, 7, uh.METRIC_L2, r2)
+    assert r == r2  # construction consumed exactly the sample and projection draws
+    use = index.space_usage()
+    assert use["index_payload_bytes"] == 3 * 7 * 4 + 3 * 4 + 2 * 24 + 7 * 2 * 4
+    assert use["unique_query_coordinates"] == 2 and use["sampled_multiplicity"] == 2
+    ws = index.make_query_workspace(4)
+    idx, dist = index.query_batch(q, 4, ws)
+    built = oracle_mod.build_hierarchical(False, REPS, probs, 1, 7, 1729)
+    cpu_idx, _ = oracle_mod.hierarchical_query_fp64(False, built, q)
+    np.testing.assert_array_equal(idx, cpu_idx)
+    _, (e_idx, e_dist) = _hier_expect(oracle_mod, False, REPS, q, probs, 1, 7, 1729)
+    np.testing.assert_array_equal(idx, e_idx)
+    np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
+    assert index.query(np.array([0.2, NAN, 0.1, INF], F32), ws) == cpu_idx[0]
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query(np.array([NAN, 0, 0, 0], F32), ws)
+
+
+@pytest.mark.parametrize("metric_l1,p", [(False, 31), (True, 17)])
+def test_hierarchical_wide_identity(uh, metric_l1, p):
+    # d = 513 (wider than one block), identity sample: each query is its own row
+    rows, d = 4, 513
+    col = np.arange(d)
+    refs = np.array([[2.0 * r + 0.01 * ((c % 7) - 3) for c in col] for r in range(rows)], F32)
+    sample = uh.make_sample(col, np.ones(d), np.ones(d))
+    cls = uh.HipHierarchicalL1AnnIndex if metric_l1 else uh.HipHierarchicalL2AnnIndex
+    index = cls(refs, None, 0, p, uh.Rng(1729), sample=sample)
+    ws = index.make_query_workspace(rows)
+    idx, _ = index.query_batch(refs, rows, ws)
+    assert list(idx) == list(range(rows))
+
+
+def test_hierarchical_l1_reference_fixture(uh, oracle_mod):
+    # cuda_hierarchical_l1_ann_index_test.cpp:50-178
+    refs = np.array([[0, 99], [5, -99]], F32)
+    probs = np.array([1.0, 0.0])
+    index = uh.HipHierarchicalL1AnnIndex(refs, probs, 16, 6, uh.Rng(1729))
+    use = index.space_usage()
+    assert use["index_payload_bytes"] == 24 + 6 * 4 + 2 * 6 * 4
+    assert use["unique_query_coordinates"] == 1 and use["sampled_multiplicity"] == 16
+    ws = index.make_query_workspace(4)
+    q = np.array([[1, 100], [4, -100], [2.5, 5], [5, 0]], F32)
+    idx, _ = index.query_batch(q, 4, ws)
+    built = oracle_mod.build_hierarchical(True, refs, probs, 16, 6, 1729)
+    cpu_idx, _ = oracle_mod.hierarchical_query_fp64(True, built, q)
+    np.testing.assert_array_equal(idx, cpu_idx)
+    assert idx[2] == 0
+    assert index.query(np.array([1, NAN], F32), ws) == cpu_idx[0]
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.query(np.array([INF, 0], F32), ws)
+
+
+@pytest.mark.parametrize("metric_l1", [False, True])
+@pytest.mark.parametrize("r,d,b,T,p", [(80, 1200, 21, 32, 16), (250, 3000, 33, 32, 64), (40, 700, 9, 8, 6)])
+def test_hierarchical_parity_bit_exact(uh, oracle_mod, metric_l1, r, d, b, T, p):
+    refs, q = _random_index_case(r * 7 + p, r, d, b)
+    probs = oracle_mod.importance_probabilities(metric_l1, refs)
+    cls = uh.HipHierarchicalL1AnnIndex if metric_l1 else uh.HipHierarchicalL2AnnIndex
+    index = cls(refs, probs, T, p, uh.Rng(11))
+    ws = index.make_query_workspace(b)
+    idx, dist = index.query_batch(q, b, ws)
+    _, (e_idx, e_dist) = _hier_expect(oracle_mod, metric_l1, refs, q, probs, T, p, 11)
+    np.testing.assert_array_equal(idx, e_idx)
+    np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
+
+
+def test_hierarchical_single_row_and_errors(uh):
+    refs = np.array([[1.0, 2.0, 3.0]], F32)
+    index = uh.HipHierarchicalL2AnnIndex(refs, np.array([1.0, 1.0, 1.0]), 2, 3, uh.Rng(3))
+    ws = index.make_query_workspace(2)
+    assert list(index.query_batch(np.ones((2, 3), F32), 2, ws)[0]) == [0, 0]
+    with pytest.raises(uh.UhannInvalidArgument):
+        uh.HipHierarchicalL2AnnIndex(refs, np.array([1.0, 1.0, 1.0]), 2, 0, uh.Rng(3))
+    with pytest.raises(uh.UhannError):
+        uh.HipHierarchicalL2AnnIndex(np.ones((2, 3), F32), np.zeros(3), 2, 3, uh.Rng(3))
+
+
+def test_async_status_reports_sampled_nan(uh):
+    import torch
+
+    sample = uh.make_sample([0, 2], [2, 1], [0.5, 4.0])
+    index = uh.HipSampledCoordinateL2AnnIndex(REPS, sample)
+    ws = index.make_query_workspace(2)
+    q = torch.tensor([[0.5, 0, 0.25, 0], [NAN, 0, 0, 0]], dtype=torch.float32, device="cuda")
+    oi = torch.zeros(2, dtype=torch.int64, device="cuda")
+    od = torch.zeros(2, dtype=torch.float32, device="cuda")
+    index.query_device_async(q, 2, ws, oi, od)
+    with pytest.raises(uh.UhannInvalidArgument):
+        ws.status()
+    ws.status()  # cleared

@@ -36,6 +36,21 @@ from ._native import (
     u64ptr,
 )
 
+from .indexes import (  # noqa: E402
+    HipExactIndex,
+    HipFlatL1AnnIndex,
+    HipFlatL2AnnIndex,
+    HipHierarchicalL1AnnIndex,
+    HipHierarchicalL2AnnIndex,
+    HipSampledCoordinateL1AnnIndex,
+    HipSampledCoordinateL2AnnIndex,
+    Rng,
+    build_coordinate_sample,
+    build_projection,
+    importance_probabilities,
+    make_sample,
+)
+
 METRIC_L2 = 0
 METRIC_L1 = 1
 STRATEGY_DIRECT = 0
@@ -47,6 +62,18 @@ __all__ = [
     "STRATEGY_DIRECT",
     "STRATEGY_GRAM_BF16",
     "HipDenseL1ScanIndex",
+    "HipExactIndex",
+    "HipFlatL1AnnIndex",
+    "HipFlatL2AnnIndex",
+    "HipHierarchicalL1AnnIndex",
+    "HipHierarchicalL2AnnIndex",
+    "HipSampledCoordinateL1AnnIndex",
+    "HipSampledCoordinateL2AnnIndex",
+    "Rng",
+    "build_coordinate_sample",
+    "build_projection",
+    "importance_probabilities",
+    "make_sample",
     "HipDenseL2ScanIndex",
     "HipExactL1Index",
     "HipExactL2Index",

@@ -92,6 +92,27 @@ def _declare(lib):
         "uhann_workspace_mixed_stats": (c_int, [c_void_p, u64p, u64p, u64p]),
         "uhann_workspace_set_timing": (c_int, [c_void_p, c_int]),
         "uhann_workspace_stage_times": (c_int, [c_void_p, f32p, c_int]),
+        "uhann_exact_create": (c_int, [c_int, f32p, c_uint64, c_uint64, c_int, POINTER(c_void_p)]),
+        "uhann_flat_create": (c_int, [c_int, f32p, c_uint64, c_uint64, c_void_p, c_uint64, c_int,
+                                      POINTER(c_void_p)]),
+        "uhann_hierarchical_create": (c_int, [c_int, f32p, c_uint64, c_uint64, c_void_p, c_uint64, f64p,
+                                              c_uint64, c_int, POINTER(c_void_p)]),
+        "uhann_index_destroy": (None, [c_void_p]),
+        "uhann_index_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), u64p, u64p, u64p, u64p, u64p]),
+        "uhann_index_enable_mixed": (c_int, [c_void_p, c_int]),
+        "uhann_index_workspace_create": (c_int, [c_void_p, c_uint64, POINTER(c_void_p)]),
+        "uhann_index_query_batch": (c_int, [c_void_p, c_void_p, f32p, c_uint64, u64p, f32p, c_int]),
+        "uhann_index_query_device_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, u64p, f32p, c_int]),
+        "uhann_index_query_device_async": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_void_p,
+                                                   c_void_p, c_int, c_void_p]),
+        "uhann_workspace_status": (c_int, [c_void_p]),
+        "uhann_rng_create": (c_int, [c_uint64, POINTER(c_void_p)]),
+        "uhann_rng_destroy": (None, [c_void_p]),
+        "uhann_rng_equal": (c_int, [c_void_p, c_void_p, POINTER(c_int)]),
+        "uhann_rng_next": (c_int, [c_void_p, u64p]),
+        "uhann_build_coordinate_sample": (c_int, [c_void_p, f64p, c_uint64, c_uint64, c_void_p, c_uint64, u64p]),
+        "uhann_build_projection": (c_int, [c_void_p, c_int, c_void_p, c_uint64, c_uint64, f64p]),
+        "uhann_importance_probabilities": (c_int, [c_int, f32p, c_uint64, c_uint64, f64p]),
     }
     optional = {}
     for name, (res, args) in sigs.items():

@@ -123,7 +123,7 @@ Workspace::Workspace(const void* owner_value, int device_value, uint64_t max_bat
     keys = DeviceBuffer<unsigned long long>(device, max_batch);
     out_idx = DeviceBuffer<uint64_t>(device, max_batch);
     out_dist = DeviceBuffer<float>(device, max_batch);
-    invalid = DeviceBuffer<int>(device, 1);
+    invalid = DeviceBuffer<int>(device, 4);
     check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking),
               "cannot create a HIP stream");
     payload_bytes = queries.bytes() + extra.bytes() + keys.bytes() + out_idx.bytes() +
@@ -320,14 +320,6 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
 }  // namespace uhann
 
 using namespace uhann;
-
-struct uhann_scan : Scan {
-    using Scan::Scan;
-};
-
-struct uhann_workspace : Workspace {
-    using Workspace::Workspace;
-};
 
 namespace {
 

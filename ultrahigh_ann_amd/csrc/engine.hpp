@@ -108,3 +108,12 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
                   uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream);
 
 }  // namespace uhann
+
+// C ABI handle types (include/uhann.h) backed by the engine objects.
+struct uhann_scan : uhann::Scan {
+    using Scan::Scan;
+};
+
+struct uhann_workspace : uhann::Workspace {
+    using Workspace::Workspace;
+};
