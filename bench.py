@@ -80,6 +80,141 @@ def cpu_baseline(refs_np, queries_np, sample):
     }, idx
 
 
+def timed(fn, steps, warmup, torch):
+    """Run fn() warmup + steps times; return (wall seconds, per-step event ms)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    events = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    t0 = time.perf_counter()
+    events[0].record(stream)
+    for i in range(steps):
+        fn()
+        events[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, np.array([events[i].elapsed_time(events[i + 1]) for i in range(steps)])
+
+
+def quality(torch, refs, queries, got_idx, exact_idx, l1):
+    """Agreement with the exact neighbour and relative distance excess
+    100 * (returned / optimum - 1) with binary64 distances (L2: square roots),
+    the reference's metrics (benchmark_metrics.cpp:302-391, README.md:351-353)."""
+    gi = torch.as_tensor(got_idx.astype(np.int64), device="cuda")
+    ei = torch.as_tensor(exact_idx.astype(np.int64), device="cuda")
+    out = []
+    for s in range(0, queries.shape[0], 512):
+        q = queries[s:s + 512].double()
+        dg = (refs[gi[s:s + 512]].double() - q)
+        de = (refs[ei[s:s + 512]].double() - q)
+        if l1:
+            a, b = dg.abs().sum(1), de.abs().sum(1)
+        else:
+            a, b = dg.pow(2).sum(1).sqrt(), de.pow(2).sum(1).sqrt()
+        out.append(torch.where(b > 0, a / b - 1.0, torch.zeros_like(a)))
+    excess = torch.cat(out) * 100.0
+    return {
+        "agreement": round(float((gi == ei).double().mean()), 5),
+        "mean_excess_pct": round(float(excess.mean()), 5),
+        "p99_excess_pct": round(float(torch.quantile(excess, 0.99)), 5),
+        "over_10pct": int((excess > 10.0).sum()),
+    }
+
+
+def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
+    """Secondary workloads at the same shape; each entry is independent."""
+    r, d = refs_np.shape
+    B = queries.shape[0]
+    res = {}
+    oi = torch.zeros(B, dtype=torch.int64, device="cuda")
+    od = torch.zeros(B, dtype=torch.float32, device="cuda")
+    steps = 2
+
+    def add(name, fn):
+        try:
+            res[name] = fn()
+            log(f"suite {name}: {res[name]}")
+        except Exception as exc:  # report, keep the headline
+            res[name] = {"error": f"{type(exc).__name__}: {exc}"}
+            log(f"suite {name} failed: {exc}")
+
+    def exact_l1():
+        index = uh.HipExactIndex(refs_np, uh.METRIC_L1)
+        ws = index.make_query_workspace(B)
+        wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od), steps, 1, torch)
+        return {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
+                "tflops_2flop": round(2.0 * r * d * B / (ms.mean() * 1e-3) / 1e12, 2)}
+
+    def streaming():
+        index = uh.HipExactIndex(refs_np, uh.METRIC_L2)
+        ws = index.make_query_workspace(64)
+        rows = {}
+        for b in (1, 4, 16):
+            wall, ms = timed(lambda: index.query_device_async(queries, b, ws, oi, od), 5, 2, torch)
+            t = float(np.median(ms)) * 1e-3
+            bytes_alg = 4.0 * r * d + 4.0 * b * d + 12.0 * b
+            rows[f"batch_{b}"] = {"queries_per_s": round(b / t, 1), "ms": round(t * 1e3, 4),
+                                  "hbm_gbps": round(bytes_alg / t / 1e9, 1),
+                                  "hbm_frac": round(bytes_alg / t / 8.0e12, 4)}
+        return rows
+
+    probs = {}
+
+    def flat(metric):
+        l1 = metric == uh.METRIC_L1
+        if metric not in probs:
+            t0 = time.perf_counter()
+            probs[metric] = uh.importance_probabilities_device(metric, refs_np)
+            log(f"GPU importance probabilities ({'L1' if l1 else 'L2'}) {time.perf_counter() - t0:.2f} s")
+        rows = {}
+        for T in (8, 32, 128):
+            cls = uh.HipFlatL1AnnIndex if l1 else uh.HipFlatL2AnnIndex
+            index = cls(refs_np, probs[metric], T, uh.Rng(args.seed))
+            use = index.space_usage()
+            ws = index.make_query_workspace(B)
+            wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od), steps, 1, torch)
+            got = oi.cpu().numpy().astype(np.uint64)
+            row = {"unique_coordinates": use["unique_query_coordinates"],
+                   "queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3)}
+            if exact_idx_l2 is not None and not l1:
+                row.update(quality(torch, refs, queries, got, exact_idx_l2, False))
+            if not l1:
+                index.enable_mixed()
+                wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od,
+                                                                  strategy=uh.STRATEGY_GRAM_BF16), steps, 1, torch)
+                row["mixed_queries_per_s"] = round(B * steps / wall, 1)
+                row["mixed_same_as_direct"] = round(float(np.mean(oi.cpu().numpy().astype(np.uint64) == got)), 5)
+            rows[f"T{T}"] = row
+        return rows
+
+    def hierarchical(metric):
+        l1 = metric == uh.METRIC_L1
+        if metric not in probs:
+            probs[metric] = uh.importance_probabilities_device(metric, refs_np)
+        rows = {}
+        cls = uh.HipHierarchicalL1AnnIndex if l1 else uh.HipHierarchicalL2AnnIndex
+        for p in (16, 32, 64):
+            index = cls(refs_np, probs[metric], 32, p, uh.Rng(args.seed))
+            ws = index.make_query_workspace(B)
+            wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od), steps, 1, torch)
+            got = oi.cpu().numpy().astype(np.uint64)
+            row = {"unique_coordinates": index.space_usage()["unique_query_coordinates"],
+                   "queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3)}
+            if exact_idx_l2 is not None and not l1:
+                row.update(quality(torch, refs, queries, got, exact_idx_l2, False))
+            rows[f"T32_p{p}"] = row
+        return rows
+
+    add("exact_l1_fp32_direct", exact_l1)
+    add("exact_l2_streaming", streaming)
+    add("flat_l2", lambda: flat(uh.METRIC_L2))
+    add("flat_l1", lambda: flat(uh.METRIC_L1))
+    add("hierarchical_l2_jl", lambda: hierarchical(uh.METRIC_L2))
+    add("hierarchical_l1_cauchy", lambda: hierarchical(uh.METRIC_L1))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +226,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=0, help="queries timed on the CPU (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-suite", action="store_true", help="skip the secondary workloads")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -180,6 +316,10 @@ def main():
         }
         log(f"CPU baseline {cpu['value']:.2f} q/s; agreement with binary64 {agreement}")
 
+    workloads = None
+    if not args.no_suite:
+        workloads = suite(uh, torch, refs, queries, refs_np, mixed_idx, args)
+
     line = {
         "metric": METRIC,
         "value": round(qps, 2),
@@ -216,6 +356,7 @@ def main():
         "agreement_fp64": agreement,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "workloads": workloads,
     }
     print(json.dumps(line), flush=True)
 

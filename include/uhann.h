@@ -274,6 +274,15 @@ int uhann_build_projection(uhann_rng* rng, int metric, const uhann_sampled_colum
 int uhann_importance_probabilities(int metric, const float* refs, uint64_t rows, uint64_t dim,
                                    double* out);
 
+/* The same probabilities computed on `device`, bit-identical to
+ * uhann_importance_probabilities (binary64 pair distances summed in the CPU
+ * order, IEEE division, order-free maximum). Device memory: rows x dim floats
+ * + rows^2 doubles. Replaces compute_l{1,2}_importance_probabilities_gpu
+ * (hnsvw25/l{1,2}/cuda_importance_sampling.cu), whose FP32 result differs
+ * from the CPU policy. */
+int uhann_importance_probabilities_device(int metric, const float* host_refs, uint64_t rows,
+                                          uint64_t dim, int device, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -245,3 +245,14 @@ def test_async_status_reports_sampled_nan(uh):
     with pytest.raises(uh.UhannInvalidArgument):
         ws.status()
     ws.status()  # cleared
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+@pytest.mark.parametrize("r,d", [(70, 1031), (130, 257), (2, 5), (9, 64)])
+def test_device_probabilities_bit_exact(uh, oracle_mod, metric, r, d):
+    g = rng(r + d + metric)
+    refs = (g.random((r, d), dtype=np.float32) * 4).astype(F32)
+    refs[r // 2] = refs[0]  # a zero-distance pair
+    gpu = uh.importance_probabilities_device(metric, refs)
+    cpu = oracle_mod.importance_probabilities(metric == 1, refs)
+    np.testing.assert_array_equal(gpu.view(np.uint64), cpu.view(np.uint64))

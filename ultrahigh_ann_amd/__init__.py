@@ -48,6 +48,7 @@ from .indexes import (  # noqa: E402
     build_coordinate_sample,
     build_projection,
     importance_probabilities,
+    importance_probabilities_device,
     make_sample,
 )
 
@@ -73,6 +74,7 @@ __all__ = [
     "build_coordinate_sample",
     "build_projection",
     "importance_probabilities",
+    "importance_probabilities_device",
     "make_sample",
     "HipDenseL2ScanIndex",
     "HipExactL1Index",

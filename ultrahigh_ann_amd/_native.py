@@ -113,6 +113,7 @@ def _declare(lib):
         "uhann_build_coordinate_sample": (c_int, [c_void_p, f64p, c_uint64, c_uint64, c_void_p, c_uint64, u64p]),
         "uhann_build_projection": (c_int, [c_void_p, c_int, c_void_p, c_uint64, c_uint64, f64p]),
         "uhann_importance_probabilities": (c_int, [c_int, f32p, c_uint64, c_uint64, f64p]),
+        "uhann_importance_probabilities_device": (c_int, [c_int, f32p, c_uint64, c_uint64, c_int, f64p]),
     }
     optional = {}
     for name, (res, args) in sigs.items():

@@ -14,6 +14,7 @@
 
 #include "common.hpp"
 #include "engine.hpp"
+#include "probabilities.hpp"
 #include "ultrahigh_ann_amd/build.hpp"
 
 namespace uab = ultrahigh_ann_amd;
@@ -164,6 +165,30 @@ int uhann_importance_probabilities(int metric, const float* refs, uint64_t rows,
         for (const auto& l : local) {
             for (uint64_t c = 0; c < dim; ++c) out[c] = std::max(out[c], l[c]);
         }
+    });
+}
+
+int uhann_importance_probabilities_device(int metric, const float* host_refs, uint64_t rows,
+                                          uint64_t dim, int device, double* out)
+{
+    return guarded([&] {
+        if (metric != UHANN_METRIC_L2 && metric != UHANN_METRIC_L1) throw std::invalid_argument("unknown metric");
+        if (out == nullptr || (rows * dim != 0 && host_refs == nullptr)) throw std::invalid_argument("null argument");
+        if (rows > 0xffffffffULL || dim > 0xffffffffULL) throw std::length_error("matrix exceeds the 32-bit range");
+        uhann::validate_finite_host(host_refs, rows * dim, "representatives");
+        if (dim == 0) return;
+        uhann::select_device(device);
+        uhann::DeviceBuffer<float> refs(device, uhann::checked_mul(rows, dim, "representatives"));
+        uhann::DeviceBuffer<double> inv(device, rows >= 2 ? uhann::checked_mul(rows, rows, "pair table") : 0);
+        uhann::DeviceBuffer<unsigned long long> bits(device, dim);
+        uhann::check_hip(hipMemcpy(refs.data(), host_refs, refs.bytes(), hipMemcpyHostToDevice),
+                         "cannot copy representatives to the HIP device");
+        uhann::kern::importance_probabilities(metric == UHANN_METRIC_L1, refs.data(), dim,
+                                              static_cast<uint32_t>(rows), static_cast<uint32_t>(dim),
+                                              inv.data(), bits.data(), nullptr);
+        uhann::check_hip(hipDeviceSynchronize(), "importance probability kernels failed");
+        uhann::check_hip(hipMemcpy(out, bits.data(), dim * sizeof(double), hipMemcpyDeviceToHost),
+                         "cannot copy probabilities from the HIP device");
     });
 }
 

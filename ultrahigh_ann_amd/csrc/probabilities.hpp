@@ -1,0 +1,16 @@
+// Launch interface of the GPU importance-probability kernels (probabilities.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace uhann::kern {
+
+// out_bits[c] <- bit pattern of p_c (binary64, the reference's sequential CPU
+// value). refs: rows x dim binary32 on the device, row stride ld. inv_scratch:
+// rows x rows doubles (only a < b is used).
+void importance_probabilities(bool l1, const float* refs, uint64_t ld, uint32_t rows, uint32_t dim,
+                              double* inv_scratch, unsigned long long* out_bits, hipStream_t stream);
+
+}  // namespace uhann::kern

@@ -93,6 +93,15 @@ def importance_probabilities(metric: int, representatives) -> np.ndarray:
     return out
 
 
+def importance_probabilities_device(metric: int, representatives, device: int = 0) -> np.ndarray:
+    """GPU importance probabilities, bit-identical to importance_probabilities()."""
+    refs = np.ascontiguousarray(representatives, dtype=np.float32)
+    out = np.zeros(refs.shape[1], np.float64)
+    check(_native.lib().uhann_importance_probabilities_device(metric, f32ptr(refs), refs.shape[0], refs.shape[1],
+                                                              device, f64ptr(out)))
+    return out
+
+
 class IndexWorkspace:
     def __init__(self, handle, owner):
         self._h = handle
