@@ -149,7 +149,8 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
     def streaming():
         index = uh.HipExactIndex(refs_np, uh.METRIC_L2)
         ws = index.make_query_workspace(64)
-        rows = {}
+        table = index.autotune(queries, 16, ws)
+        rows = {"autotuned": {f"<= {t['max_batch']}": [t["rpt"], t["tq"], t["tr"], t["group_q"]] for t in table}}
         for b in (1, 4, 16):
             wall, ms = timed(lambda: index.query_device_async(queries, b, ws, oi, od), 5, 2, torch)
             t = float(np.median(ms)) * 1e-3
@@ -173,10 +174,12 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
             index = cls(refs_np, probs[metric], T, uh.Rng(args.seed))
             use = index.space_usage()
             ws = index.make_query_workspace(B)
+            tuned = index.autotune(queries, B, ws)[-1]
             wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od), steps, 1, torch)
             got = oi.cpu().numpy().astype(np.uint64)
             row = {"unique_coordinates": use["unique_query_coordinates"],
-                   "queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3)}
+                   "queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
+                   "tile": [tuned["rpt"], tuned["tq"], tuned["tr"], tuned["group_q"]]}
             if exact_idx_l2 is not None and not l1:
                 row.update(quality(torch, refs, queries, got, exact_idx_l2, False))
             if not l1:
@@ -197,6 +200,8 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
         for p in (16, 32, 64):
             index = cls(refs_np, probs[metric], 32, p, uh.Rng(args.seed))
             ws = index.make_query_workspace(B)
+            if not l1:
+                index.autotune(queries, B, ws)
             wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od), steps, 1, torch)
             got = oi.cpu().numpy().astype(np.uint64)
             row = {"unique_coordinates": index.space_usage()["unique_query_coordinates"],

@@ -124,8 +124,10 @@ int uhann_scan_query_device_async(const uhann_scan* scan, uhann_workspace* ws,
 
 /* As uhann_scan_query_device_async with an explicit register tile: rpt
  * chains per thread (1, 2 or 4), tq x tr (query, reference) pairs per block,
- * group_q query tiles per reference sweep. Returns INVALID_ARGUMENT for a tile
- * that is not compiled in. No counterpart in the reference: used by the
+ * group_q query tiles per reference sweep; rpt = 0 selects the
+ * pair-per-thread scan for rows of at most 4096 coordinates with tq (1, 2, 4)
+ * queries per thread. Returns INVALID_ARGUMENT for a tile that is not
+ * compiled in. No counterpart in the reference: used by the
  * autotuner and by the tests that cover every tile. */
 int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
                                  const float* device_queries, uint64_t ld_queries, uint64_t count,
@@ -282,6 +284,23 @@ int uhann_importance_probabilities(int metric, const float* refs, uint64_t rows,
  * from the CPU policy. */
 int uhann_importance_probabilities_device(int metric, const float* host_refs, uint64_t rows,
                                           uint64_t dim, int device, double* out);
+
+/* Autotuner of the direct scan: times every compiled kernel shape (the
+ * tiled register tiles and the pair-per-thread scan) with HIP events for
+ * batches of 1, 2, 4, ... up to max_batch device queries (row stride
+ * ld_queries) and keeps the fastest per batch bucket in the handle; later
+ * DIRECT calls use it. Writes one row of 6 ints per tuned bucket to table
+ * (bucket b covers batches 2^(b-1)+1 .. 2^b; rpt, tq, tr, group_q;
+ * microseconds) up to capacity rows, and the row count to *count. Call before
+ * sharing the handle between threads. No counterpart in the reference. */
+int uhann_scan_autotune(uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                        uint64_t ld_queries, uint64_t max_batch, int* table, int capacity, int* count);
+
+/* uhann_scan_autotune for the direct scan inside an exact, flat or
+ * hierarchical-L2 index, fed with max_batch full-width device queries (the
+ * gather and projection run first). INVALID_ARGUMENT for hierarchical L1. */
+int uhann_index_autotune(uhann_index* index, uhann_workspace* ws, const float* device_queries,
+                         uint64_t ld_queries, uint64_t max_batch, int* table, int capacity, int* count);
 
 #ifdef __cplusplus
 }

@@ -256,3 +256,18 @@ def test_device_probabilities_bit_exact(uh, oracle_mod, metric, r, d):
     gpu = uh.importance_probabilities_device(metric, refs)
     cpu = oracle_mod.importance_probabilities(metric == 1, refs)
     np.testing.assert_array_equal(gpu.view(np.uint64), cpu.view(np.uint64))
+
+
+def test_flat_autotune_keeps_bits(uh, oracle_mod):
+    import torch
+
+    refs, q = _random_index_case(21, 120, 2000, 33)
+    probs = oracle_mod.importance_probabilities(False, refs)
+    index = uh.HipFlatL2AnnIndex(refs, probs, 32, uh.Rng(3))
+    ws = index.make_query_workspace(33)
+    before, bd = index.query_batch(q, 33, ws)
+    table = index.autotune(torch.from_numpy(q).cuda(), 33, ws)
+    assert table
+    after, ad = index.query_batch(q, 33, ws)
+    np.testing.assert_array_equal(before, after)
+    np.testing.assert_array_equal(bd.view(np.uint32), ad.view(np.uint32))

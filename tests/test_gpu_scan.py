@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 F32 = np.float32
 
 TILES = [(1, 8, 8, 8), (1, 12, 12, 8), (1, 16, 8, 4), (1, 8, 16, 2), (1, 4, 16, 1), (1, 2, 16, 1),
-         (1, 1, 16, 1), (4, 1, 8, 1), (4, 2, 8, 1), (4, 4, 8, 1), (4, 8, 4, 2)]
+         (1, 1, 16, 1), (4, 1, 8, 1), (4, 2, 8, 1), (4, 4, 8, 1), (4, 8, 4, 2),
+         (0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0)]
 
 
 def test_exact_l2_reference_fixture(uh):
@@ -95,7 +96,10 @@ def _data(seed, r, d, b, dup=True):
     [(37, 1000, 19), (5, 3, 7), (64, 256, 33), (130, 4097, 5), (1, 70, 3), (300, 60660, 2)],
 )
 @pytest.mark.parametrize("metric", ["l2", "l1", "l1w"])
-def test_parity_bit_exact(uh, oracle_mod, metric, r, d, b):
+@pytest.mark.parametrize("pairs", ["auto", "off"])
+def test_parity_bit_exact(uh, oracle_mod, metric, r, d, b, pairs, monkeypatch):
+    if pairs == "off":
+        monkeypatch.setenv("UHANN_PAIRS_MAX_DIM", "0")
     refs, q = _data(r * 31 + d, r, d, b)
     if metric == "l2":
         index = uh.HipDenseL2ScanIndex(refs)
@@ -115,7 +119,7 @@ def test_parity_bit_exact(uh, oracle_mod, metric, r, d, b):
 
 
 @pytest.mark.parametrize("tile", TILES)
-@pytest.mark.parametrize("d", [1000, 515, 4096])
+@pytest.mark.parametrize("d", [1000, 515, 4096, 33, 257])
 def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
     import torch
 
@@ -173,3 +177,21 @@ def test_zero_dimension_and_overflow(uh, oracle_mod):
     idx, dist = index.query_batch(np.array([-3e38, 3e38], F32), 1, ws)
     exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, np.array([[-3e38, 3e38]], F32)))
     assert idx[0] == exp_idx[0] and np.isinf(dist[0]) == np.isinf(exp_dist[0])
+
+
+@pytest.mark.parametrize("d", [300, 5000])
+def test_autotune_keeps_bits(uh, oracle_mod, d):
+    import torch
+
+    r, b = 200, 40
+    refs, q = _data(11 + d, r, d, b)
+    index = uh.HipDenseL2ScanIndex(refs)
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    table = index.autotune(dq, b, ws)
+    assert len(table) >= 6 and all(row["us"] > 0 for row in table)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q))
+    for n in (1, 3, 17, 40):
+        idx, dist = index.query_device_batch(dq[:n], n, ws)
+        np.testing.assert_array_equal(idx, exp_idx[:n])
+        np.testing.assert_array_equal(dist.view(np.uint32), exp_dist[:n].view(np.uint32))

@@ -206,6 +206,17 @@ class _DenseScan:
         check(_native.lib().uhann_scan_info(self._h, None, None, None, ctypes.byref(d)))
         return d.value
 
+    def autotune(self, device_queries, max_batch: int, workspace: QueryWorkspace, ld_queries=None):
+        """Time every direct-scan kernel per batch bucket; returns the table."""
+        ptr = device_queries.data_ptr() if hasattr(device_queries, "data_ptr") else int(device_queries)
+        table = np.zeros((24, 6), np.int32)
+        n = c_int()
+        check(_native.lib().uhann_scan_autotune(
+            self._h, workspace.handle, c_void_p(ptr), self.dim if ld_queries is None else ld_queries, max_batch,
+            table.ctypes.data_as(ctypes.POINTER(c_int)), 24, ctypes.byref(n)))
+        return [{"max_batch": 1 << int(r[0]), "rpt": int(r[1]), "tq": int(r[2]), "tr": int(r[3]),
+                 "group_q": int(r[4]), "us": int(r[5])} for r in table[: n.value]]
+
     def enable_mixed(self, strategy=STRATEGY_GRAM_BF16):
         """Prepare the BF16 reference copy for the mixed-precision strategy."""
         check(_native.lib().uhann_scan_enable_mixed(self._h, strategy))

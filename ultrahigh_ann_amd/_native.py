@@ -114,6 +114,10 @@ def _declare(lib):
         "uhann_build_projection": (c_int, [c_void_p, c_int, c_void_p, c_uint64, c_uint64, f64p]),
         "uhann_importance_probabilities": (c_int, [c_int, f32p, c_uint64, c_uint64, f64p]),
         "uhann_importance_probabilities_device": (c_int, [c_int, f32p, c_uint64, c_uint64, c_int, f64p]),
+        "uhann_scan_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, POINTER(c_int), c_int,
+                                        POINTER(c_int)]),
+        "uhann_index_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, POINTER(c_int), c_int,
+                                         POINTER(c_int)]),
     }
     optional = {}
     for name, (res, args) in sigs.items():

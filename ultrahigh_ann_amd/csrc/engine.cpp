@@ -92,6 +92,17 @@ Scan::Scan(int metric_value, const float* host_refs, uint64_t rows_value, uint64
         check_hip(hipMemcpy(refs.data(), staged.data(), refs.bytes(), hipMemcpyHostToDevice),
                   "cannot copy representatives to the HIP device");
     }
+    if (dim != 0 && dim <= kern::scan_pairs_max_dim()) {
+        // Short rows also get a transposed copy (dim x rows) for the
+        // pair-per-thread scan, whose loads are then contiguous across a wave.
+        std::vector<float> t(static_cast<std::size_t>(rows * dim));
+        for (uint64_t r = 0; r < rows; ++r) {
+            for (uint64_t c = 0; c < dim; ++c) t[c * rows + r] = host_refs[r * dim + c];
+        }
+        refs_t = DeviceBuffer<float>(device, t.size());
+        check_hip(hipMemcpy(refs_t.data(), t.data(), refs_t.bytes(), hipMemcpyHostToDevice),
+                  "cannot copy transposed representatives to the HIP device");
+    }
     if (host_weights != nullptr && dim != 0) {
         weights = DeviceBuffer<float>(device, ld);
         std::vector<float> staged(static_cast<std::size_t>(ld), 0.0F);
@@ -151,6 +162,83 @@ Workspace::~Workspace()
     }
 }
 
+// Rows up to this length use the pair-per-thread scan (UHANN_PAIRS_MAX_DIM
+// overrides it for tuning).
+uint32_t pairs_max_dim()
+{
+    const char* env = std::getenv("UHANN_PAIRS_MAX_DIM");
+    if (env != nullptr) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v >= 0) return static_cast<uint32_t>(std::min<long>(v, kern::scan_pairs_max_dim()));
+    }
+    return 4096;
+}
+
+int batch_bucket(uint64_t nq)
+{
+    int b = 0;
+    while (b < 23 && (uint64_t{1} << b) < nq) ++b;
+    return b;
+}
+
+void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, bool padded,
+                   uint64_t max_batch, hipStream_t stream)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    check_hip(hipEventCreate(&e0), "cannot create a HIP event");
+    check_hip(hipEventCreate(&e1), "cannot create a HIP event");
+    try {
+        const bool aligned4 = (ld_q % 4U == 0) && (reinterpret_cast<uintptr_t>(queries) % 16U == 0) &&
+                              (scan.dim % 4U == 0 || padded);
+        std::vector<kern::ScanTile> cands;
+        for (const kern::ScanTile& t : kern::scan_tiles()) {
+            if (t.rpt == 1 || aligned4) cands.push_back(t);
+        }
+        if (scan.refs_t.bytes() != 0 && scan.dim != 0) {
+            for (int tq : {1, 2, 4}) cands.push_back(kern::ScanTile{0, tq, 0, 0});
+        }
+        for (uint64_t nq = 1;; nq *= 2) {
+            const uint64_t n = std::min(nq, max_batch);
+            float best = -1.0F;
+            kern::ScanTile best_tile{};
+            for (const kern::ScanTile& t : cands) {
+                // Tiles much taller than the batch only repeat clamped rows.
+                if (t.rpt != 0 && static_cast<uint64_t>(t.tq) > 2 * n + 1) continue;
+                float ms_min = -1.0F;
+                for (int rep = 0; rep < 3; ++rep) {
+                    check_hip(hipEventRecord(e0, stream), "cannot record a HIP event");
+                    scan_launch(scan, queries, ld_q, n, ws.keys.data(), ws.out_idx.data(), ws.out_dist.data(),
+                                stream, padded, &t);
+                    check_hip(hipEventRecord(e1, stream), "cannot record a HIP event");
+                    check_hip(hipEventSynchronize(e1), "autotune run failed");
+                    float ms = 0.0F;
+                    check_hip(hipEventElapsedTime(&ms, e0, e1), "cannot read a HIP event");
+                    if (rep > 0 && (ms_min < 0.0F || ms < ms_min)) ms_min = ms;
+                    // Clearly slower than the best so far: stop timing it.
+                    if (rep == 0 && best > 0.0F && ms > 2.0F * best) break;
+                }
+                if (ms_min > 0.0F && (best < 0.0F || ms_min < best)) {
+                    best = ms_min;
+                    best_tile = t;
+                }
+            }
+            if (best > 0.0F) {
+                Scan::Tuned& slot = scan.tuned[batch_bucket(n)];
+                slot.set = true;
+                slot.tile = best_tile;
+                slot.ms = best;
+            }
+            if (n >= max_batch) break;
+        }
+    } catch (...) {
+        static_cast<void>(hipEventDestroy(e0));
+        static_cast<void>(hipEventDestroy(e1));
+        throw;
+    }
+    static_cast<void>(hipEventDestroy(e0));
+    static_cast<void>(hipEventDestroy(e1));
+}
+
 kern::ScanTile choose_tile(uint64_t nq, bool aligned4)
 {
     kern::ScanTile t;
@@ -189,6 +277,24 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
                           (reinterpret_cast<uintptr_t>(queries) % 16U == 0) &&
                           (scan.dim % 4U == 0 || queries_padded);
     kern::ScanTile tile = forced_tile != nullptr ? *forced_tile : choose_tile(count, aligned4);
+    // Short rows: the pair-per-thread kernel (tile.rpt == 0 selects it, tq
+    // queries per thread) unless a tiled shape is forced or tuned.
+    const bool pairs_ok = scan.refs_t.bytes() != 0 && a.dim != 0;
+    const Scan::Tuned& tuned = scan.tuned[batch_bucket(count)];
+    if (forced_tile == nullptr && tuned.set && (tuned.tile.rpt <= 1 || aligned4)) {
+        tile = tuned.tile;
+    } else if (forced_tile == nullptr && pairs_ok && a.dim <= pairs_max_dim() && count >= 4) {
+        tile = kern::ScanTile{0, 4, 0, 0};
+    }
+    if (tile.rpt == 0) {
+        if (!pairs_ok) throw std::invalid_argument("pair scan needs rows of at most 4096 coordinates");
+        kern::fill_keys(keys, a.nq, stream);
+        kern::scan_pairs(scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1,
+                         /*weight_fma=*/true, scan.refs_t.data(), scan.rows, a.rows, queries, ld_q, a.nq,
+                         a.weights, a.dim, keys, tile.tq, stream);
+        kern::finalize_keys(keys, a.nq, out_idx, out_dist, stream);
+        return;
+    }
     if (tile.rpt > 1 && !aligned4) {
         throw std::invalid_argument(
             "vector scan tile needs 16-byte aligned, padded query rows");
@@ -504,7 +610,8 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
         checked_ws(scan, ws, count);
         const kern::ScanTile tile{rpt, tq, tr, group_q};
-        if (!kern::scan_tile_supported(tile)) throw std::invalid_argument("unsupported scan tile");
+        const bool pair_tile = rpt == 0 && (tq == 1 || tq == 2 || tq == 4);
+        if (!pair_tile && !kern::scan_tile_supported(tile)) throw std::invalid_argument("unsupported scan tile");
         if (count == 0) return;
         if (scan->dim != 0 && device_queries == nullptr) {
             throw std::invalid_argument("device query pointer must not be null");
@@ -513,6 +620,36 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
         select_device(scan->device);
         scan_launch(*scan, device_queries, ld_queries, count, ws->keys.data(), device_out_idx,
                     device_out_dist, static_cast<hipStream_t>(stream), false, &tile);
+    });
+}
+
+int uhann_scan_autotune(uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                        uint64_t ld_queries, uint64_t max_batch, int* table, int capacity, int* count)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        checked_ws(scan, ws, max_batch);
+        if (max_batch == 0) throw std::invalid_argument("autotune batch must be positive");
+        if (scan->dim != 0 && device_queries == nullptr) throw std::invalid_argument("device query pointer must not be null");
+        if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
+        select_device(scan->device);
+        autotune_scan(*scan, *ws, device_queries, ld_queries, false, max_batch, ws->stream);
+        int n = 0;
+        for (int b = 0; b < 24; ++b) {
+            const Scan::Tuned& t = scan->tuned[b];
+            if (!t.set) continue;
+            if (table != nullptr && n < capacity) {
+                int* row = table + 6 * n;
+                row[0] = b;
+                row[1] = t.tile.rpt;
+                row[2] = t.tile.tq;
+                row[3] = t.tile.tr;
+                row[4] = t.tile.group_q;
+                row[5] = static_cast<int>(t.ms * 1000.0F);  // microseconds
+            }
+            ++n;
+        }
+        if (count != nullptr) *count = n;
     });
 }
 

@@ -42,10 +42,20 @@ struct Scan {
     uint64_t ld = 0;
     bool weighted = false;
     DeviceBuffer<float> refs;
+    DeviceBuffer<float> refs_t;  // dim x rows copy for short rows (pair-per-thread scan)
     DeviceBuffer<float> weights;
     uint64_t payload_bytes = 0;
     std::string device_name;
     std::unique_ptr<GramRefs> gram;  // set by enable_mixed()
+
+    // Autotuned direct-scan kernels per batch bucket b (batches of
+    // 2^(b-1)+1 .. 2^b queries); empty entries fall back to the heuristic.
+    struct Tuned {
+        bool set = false;
+        kern::ScanTile tile;
+        float ms = 0.0F;
+    };
+    Tuned tuned[24];
 
     void enable_mixed();
 };
@@ -93,6 +103,14 @@ struct Workspace {
 };
 
 kern::ScanTile choose_tile(uint64_t nq, bool aligned4);
+uint32_t pairs_max_dim();
+int batch_bucket(uint64_t nq);
+
+// Times every candidate direct-scan kernel for batches of 1, 2, 4, ... up to
+// max_batch queries (row stride ld_q at `queries`) and records the fastest per
+// bucket in scan.tuned.
+void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, bool padded,
+                   uint64_t max_batch, hipStream_t stream);
 
 // fill keys -> direct scan -> finalize, all on `stream`. queries_padded
 // says every query row is readable up to the next multiple of 4 floats past

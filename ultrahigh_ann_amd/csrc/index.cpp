@@ -427,6 +427,58 @@ int uhann_index_query_device_async(const uhann_index* ix, uhann_workspace* ws, c
     });
 }
 
+int uhann_index_autotune(uhann_index* ix, uhann_workspace* ws, const float* device_queries, uint64_t ld_queries,
+                         uint64_t max_batch, int* table, int capacity, int* count)
+{
+    return guarded([&] {
+        if (ix == nullptr) throw std::invalid_argument("index handle must not be null");
+        checked_ws(ix, ws, max_batch);
+        if (max_batch == 0) throw std::invalid_argument("autotune batch must be positive");
+        if (ix->dim != 0 && device_queries == nullptr) throw std::invalid_argument("device query pointer must not be null");
+        if (ld_queries < ix->dim) throw std::invalid_argument("query row stride is below dim");
+        if (!ix->scan) throw std::invalid_argument("this index has no direct scan to tune");
+        select_device(ix->device);
+        hipStream_t s = ws->stream;
+        const uint32_t n = static_cast<uint32_t>(max_batch);
+        const float* q = device_queries;
+        uint64_t ld = ld_queries;
+        bool padded = false;
+        if (ix->kind == UHANN_KIND_FLAT) {
+            kern::gather_columns(device_queries, ld_queries, n, ix->cols.data(), ix->scales.data(),
+                                 static_cast<uint32_t>(ix->k), ws->extra.data(), ix->k_ld, ws->invalid.data(), s);
+            q = ws->extra.data();
+            ld = ix->k_ld;
+            padded = true;
+        } else if (ix->kind == UHANN_KIND_HIERARCHICAL) {
+            float* projected = ws->extra.data() + max_batch * ix->k_ld;
+            kern::gather_columns(device_queries, ld_queries, n, ix->cols.data(), nullptr, static_cast<uint32_t>(ix->k),
+                                 ws->extra.data(), ix->k_ld, ws->invalid.data(), s);
+            kern::project_fp32(ws->extra.data(), ix->k_ld, n, static_cast<uint32_t>(ix->k), ix->proj.data(), ix->k_ld,
+                               static_cast<uint32_t>(ix->p), projected, ix->p_ld, ws->invalid.data() + 1, s);
+            q = projected;
+            ld = ix->p_ld;
+            padded = true;
+        }
+        autotune_scan(*ix->scan, *ws, q, ld, padded, max_batch, s);
+        int rows = 0;
+        for (int b = 0; b < 24; ++b) {
+            const Scan::Tuned& t = ix->scan->tuned[b];
+            if (!t.set) continue;
+            if (table != nullptr && rows < capacity) {
+                int* row = table + 6 * rows;
+                row[0] = b;
+                row[1] = t.tile.rpt;
+                row[2] = t.tile.tq;
+                row[3] = t.tile.tr;
+                row[4] = t.tile.group_q;
+                row[5] = static_cast<int>(t.ms * 1000.0F);
+            }
+            ++rows;
+        }
+        if (count != nullptr) *count = rows;
+    });
+}
+
 int uhann_workspace_status(uhann_workspace* ws)
 {
     return guarded([&] {

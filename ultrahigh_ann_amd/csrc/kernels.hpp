@@ -6,6 +6,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace uhann::kern {
 
@@ -41,11 +42,21 @@ struct ScanTile {
 };
 
 bool scan_tile_supported(const ScanTile& tile);
+// Every tiled shape compiled in (candidates of the autotuner).
+const std::vector<ScanTile>& scan_tiles();
 void fill_keys(unsigned long long* keys, uint32_t n, hipStream_t stream);
 void scan_direct(Metric metric, bool weight_fma, const ScanArgs& args,
                  const ScanTile& tile, hipStream_t stream);
 void finalize_keys(const unsigned long long* keys, uint32_t n, uint64_t* out_idx,
                    float* out_dist, hipStream_t stream);
+
+// Pair-per-thread direct scan for dim <= scan_pairs_max_dim(): refs_t is the
+// transposed (dim x rows, row stride ld_t) reference matrix; tq (1, 2, 4)
+// queries per block. Same keys contract as scan_direct, same bits.
+uint32_t scan_pairs_max_dim();
+void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld_t, uint32_t rows,
+                const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,
+                unsigned long long* keys, int tq, hipStream_t stream);
 
 // Gather of sampled coordinates: out[q][j] = src[q][cols[j]] (* scales[j]
 // when scales is non-null), for q < nq, j < k. Sets *invalid to 1 when a

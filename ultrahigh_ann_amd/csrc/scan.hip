@@ -31,6 +31,7 @@
 
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -220,6 +221,119 @@ scan_direct_kernel(ScanArgs a, uint32_t tiles_q, uint32_t tiles_r, uint32_t grou
     }
 }
 
+
+// Pair-per-thread form for short rows (flat |I|, hierarchical p): the
+// 256-chain tree costs 255 adds per pair whatever the row length, and in the
+// tiled kernel above it is a cross-lane tree, which dominates when a chain
+// holds one or two terms. Here each thread owns one reference row and TQ
+// queries and evaluates the whole reference order in registers: for each
+// group w of 32 chains, chain 32w+i sums columns 32w+i, 32w+i+256, ... in
+// order, then the 32-lane tree (offsets 16..1) gives W_w, and
+// ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7)) the distance -- the same additions as
+// the reference, so the same bits. References are read from a transposed
+// (dim x rows) copy, so a wave's loads are contiguous; queries (and L1
+// weights) sit in LDS and are broadcast.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__global__ void __launch_bounds__(256)
+scan_pairs_kernel(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
+                  const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
+                  const float* __restrict__ weights, uint32_t dim, unsigned long long* __restrict__ keys)
+{
+    extern __shared__ float smem[];
+    float* qs = smem;              // TQ x dim
+    float* ws = smem + TQ * dim;   // dim (weighted L1)
+    const uint32_t q0 = blockIdx.y * TQ;
+    const int t = threadIdx.x;
+    for (uint32_t e = t; e < TQ * dim; e += 256) {
+        const uint32_t qi = e / dim;
+        const uint32_t c = e % dim;
+        qs[e] = queries[static_cast<uint64_t>(min(q0 + qi, nq - 1U)) * ld_q + c];
+    }
+    if constexpr (WEIGHTED) {
+        for (uint32_t c = t; c < dim; c += 256) ws[c] = weights[c];
+    }
+    __syncthreads();
+    const uint32_t r = blockIdx.x * 256 + t;
+    const uint32_t rr = min(r, rows - 1U);
+    // Groups are visited in bit-reversed order 0,4 | 2,6 | 1,5 | 3,7, so pass
+    // p of 4 produces (W0+W4), (W2+W6), (W1+W5), (W3+W7) in turn and the
+    // distance is (P0 + P1) + (P2 + P3): the reference's final tree with only
+    // four partial sums live (the loops stay rolled to keep registers low).
+    float P[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[q][k] = 0.0F;
+#pragma unroll 1
+    for (int pass = 0; pass < 4; ++pass) {
+        float pair_sum[TQ];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const int g = 2 * pass + h;
+            const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
+            float v[TQ][32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) v[q][i] = 0.0F;
+            // Term k of chain 32w+i is column 32w + i + 256k: all 32 chains of
+            // the group load term k together (32 independent loads in flight),
+            // and each chain still adds its terms in increasing k.
+            for (uint32_t base = static_cast<uint32_t>(32 * w); base < dim; base += kChains) {
+                float rv[32];
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const uint32_t c = min(base + i, dim - 1U);
+                    rv[i] = refs_t[static_cast<uint64_t>(c) * ld_t + rr];
+                }
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const uint32_t c = base + i;
+                    if (c < dim) {
+                        const float wv = WEIGHTED ? ws[c] : 0.0F;
+#pragma unroll
+                        for (int q = 0; q < TQ; ++q) {
+                            v[q][i] = accumulate<METRIC, WEIGHTED, WFMA>(v[q][i], rv[i], qs[q * dim + c], wv);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) {
+#pragma unroll
+                for (int off = 16; off >= 1; off /= 2) {
+#pragma unroll
+                    for (int i = 0; i < off; ++i) v[q][i] = v[q][i] + v[q][i + off];
+                }
+                pair_sum[q] = h == 0 ? v[q][0] : pair_sum[q] + v[q][0];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) P[q][k] = pass == k ? pair_sum[q] : P[q][k];
+    }
+    __shared__ unsigned long long best[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+        const float dist = (P[q][0] + P[q][1]) + (P[q][2] + P[q][3]);
+        unsigned long long key = r < rows
+                                     ? (static_cast<unsigned long long>(__float_as_uint(dist)) << 32) | r
+                                     : ~0ULL;
+        for (int off = 32; off >= 1; off /= 2) {
+            const unsigned long long o = __shfl_down(key, off);
+            key = o < key ? o : key;
+        }
+        if (t % 64 == 0) best[q][t / 64] = key;
+    }
+    __syncthreads();
+    if (t < TQ && q0 + t < nq) {
+        unsigned long long b = best[t][0];
+        for (int w = 1; w < 4; ++w) b = best[t][w] < b ? best[t][w] : b;
+        atomicMin(keys + q0 + t, b);
+    }
+}
+
 __global__ void fill_keys_kernel(unsigned long long* keys, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,6 +378,13 @@ ScanFn pick(int rpt, int tq, int tr)
     UHANN_TILE(4, 2, 8)
     UHANN_TILE(4, 4, 8)
     UHANN_TILE(4, 8, 4)
+    UHANN_TILE(1, 1, 4)
+    UHANN_TILE(1, 2, 8)
+    UHANN_TILE(1, 4, 8)
+    UHANN_TILE(1, 16, 4)
+    UHANN_TILE(4, 1, 4)
+    UHANN_TILE(4, 2, 4)
+    UHANN_TILE(4, 4, 4)
 #undef UHANN_TILE
     return nullptr;
 }
@@ -277,6 +398,16 @@ ScanFn select_scan(Metric metric, bool weighted, bool wfma, const ScanTile& t)
 }
 
 }  // namespace
+
+const std::vector<ScanTile>& scan_tiles()
+{
+    static const std::vector<ScanTile> tiles = {
+        {1, 8, 8, 8},  {1, 12, 12, 8}, {1, 16, 8, 4}, {1, 8, 16, 2}, {1, 4, 16, 1}, {1, 2, 16, 1},
+        {1, 1, 16, 1}, {4, 1, 8, 1},   {4, 2, 8, 1},  {4, 4, 8, 1},  {4, 8, 4, 2},  {1, 1, 4, 1},
+        {1, 2, 8, 1},  {1, 4, 8, 1},   {1, 16, 4, 4}, {4, 1, 4, 1},  {4, 2, 4, 1},  {4, 4, 4, 1},
+    };
+    return tiles;
+}
 
 bool scan_tile_supported(const ScanTile& tile)
 {
@@ -312,6 +443,44 @@ void scan_direct(Metric metric, bool weight_fma, const ScanArgs& args, const Sca
     hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kChains / tile.rpt), 0,
                        stream, args, tiles_q, tiles_r, group_q);
     check_launch("direct scan kernel");
+}
+
+uint32_t scan_pairs_max_dim() { return 4096; }
+
+void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld_t, uint32_t rows,
+                const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,
+                unsigned long long* keys, int tq, hipStream_t stream)
+{
+    if (nq == 0 || rows == 0) return;
+    if (dim > scan_pairs_max_dim()) throw std::invalid_argument("pair scan supports dim <= 4096");
+    const bool weighted = weights != nullptr;
+    const size_t lds = (static_cast<size_t>(tq) * dim + (weighted ? dim : 0)) * sizeof(float);
+    const dim3 grid((rows + 255) / 256, (nq + tq - 1) / tq);
+    if (grid.y > 65535U) throw std::length_error("pair scan batch exceeds the grid limit");
+#define UHANN_PAIRS(M, W, F, Q)                                                                   \
+    hipLaunchKernelGGL((scan_pairs_kernel<M, W, F, Q>), grid, dim3(256), lds, stream, refs_t, ld_t, \
+                       rows, queries, ld_q, nq, weights, dim, keys)
+    const int m = metric == Metric::l2 ? 0 : 1;
+    if (tq == 1) {
+        if (m == 0) UHANN_PAIRS(0, false, false, 1);
+        else if (!weighted) UHANN_PAIRS(1, false, false, 1);
+        else if (weight_fma) UHANN_PAIRS(1, true, true, 1);
+        else UHANN_PAIRS(1, true, false, 1);
+    } else if (tq == 2) {
+        if (m == 0) UHANN_PAIRS(0, false, false, 2);
+        else if (!weighted) UHANN_PAIRS(1, false, false, 2);
+        else if (weight_fma) UHANN_PAIRS(1, true, true, 2);
+        else UHANN_PAIRS(1, true, false, 2);
+    } else if (tq == 4) {
+        if (m == 0) UHANN_PAIRS(0, false, false, 4);
+        else if (!weighted) UHANN_PAIRS(1, false, false, 4);
+        else if (weight_fma) UHANN_PAIRS(1, true, true, 4);
+        else UHANN_PAIRS(1, true, false, 4);
+    } else {
+        throw std::invalid_argument("pair scan supports 1, 2 or 4 queries per block");
+    }
+#undef UHANN_PAIRS
+    check_launch("pair scan kernel");
 }
 
 }  // namespace uhann::kern

@@ -170,6 +170,17 @@ class _Index:
     def enable_mixed(self, strategy=STRATEGY_GRAM_BF16):
         check(_native.lib().uhann_index_enable_mixed(self._h, strategy))
 
+    def autotune(self, device_queries, max_batch: int, workspace: IndexWorkspace, ld_queries=None):
+        """Tune the direct scan per batch bucket on max_batch device queries."""
+        ptr = device_queries.data_ptr() if hasattr(device_queries, "data_ptr") else int(device_queries)
+        table = np.zeros((24, 6), np.int32)
+        n = c_int()
+        check(_native.lib().uhann_index_autotune(
+            self._h, workspace.handle, c_void_p(ptr), self.dim if ld_queries is None else ld_queries, max_batch,
+            table.ctypes.data_as(ctypes.POINTER(c_int)), 24, ctypes.byref(n)))
+        return [{"max_batch": 1 << int(r[0]), "rpt": int(r[1]), "tq": int(r[2]), "tr": int(r[3]),
+                 "group_q": int(r[4]), "us": int(r[5])} for r in table[: n.value]]
+
     def make_query_workspace(self, maximum_batch_size: int) -> IndexWorkspace:
         h = c_void_p()
         check(_native.lib().uhann_index_workspace_create(self._h, maximum_batch_size, ctypes.byref(h)))
