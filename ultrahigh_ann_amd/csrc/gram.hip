@@ -10,13 +10,14 @@
 //    operands double-buffered in LDS with a register prefetch of the next K
 //    step.
 // 3. select_rerank (one block per query):
-//    * per-pair bound tau(q,r) on |approx - exact|: bf16 rounding of both
-//      operands is a sum of independent zero-mean terms |c_i| <= u_b |q_i v_i|
-//      (u_b = 2^-8), so by Hoeffding |<q,v> - <q^,v^>| <= K u_b ||q||_4
-//      sqrt(||v||_4^2 + ||v^||_4^2) except with probability 2 exp(-K^2/2)
-//      (K = 7: 4.6e-11 per pair); the FP32 accumulation over ceil(d/16) MFMA
-//      steps adds K u_32 sqrt(ceil(d/16)) ||q^||_2 ||v^||_2, and the norms and
-//      the three-term epilogue add 4 u_32 (||q||^2 + ||v||^2);
+//    * per-pair bound tau(q,r) on |approx - exact|: the bf16 rounding error
+//      of the inner product is sum_i (q_i dv_i + v^_i dq_i); with the rounding
+//      signs modelled as independent and symmetric, Hoeffding bounds it by
+//      K (||q||_4 ||dv||_4 + ||v^||_4 ||dq||_4) except with probability
+//      2 exp(-K^2/2) (K = 7: 4.6e-11 per pair), where dv, dq are the rows'
+//      actual rounding errors (stored per row); the FP32 accumulation over
+//      ceil(d/16) MFMA steps adds K u_32 sqrt(ceil(d/16)) ||q^||_2 ||v^||_2,
+//      and the norms and the three-term epilogue add 4 u_32 (||q||^2 + ||v||^2);
 //    * U = min_r (approx + tau); candidates = {r : approx - tau <= U}. The
 //      exact argmin is a candidate, and no non-candidate can tie it;
 //    * the candidates are re-ranked with the reference's binary32 order
@@ -46,7 +47,6 @@ constexpr int kPad = 8;                 // bf16 elements of row padding in LDS
 constexpr int kLdsRow = kBK + kPad;     // 72 bf16 = 144 bytes
 constexpr float kHoeffdingK = 7.0F;
 constexpr float kU32 = 5.9604645e-08F;  // 2^-24
-constexpr float kUbf16 = 0.00390625F;   // 2^-8
 
 void check_launch(const char* what)
 {
@@ -81,28 +81,33 @@ to_bf16_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t row
     if (row >= rows) return;
     const float* s = src + static_cast<uint64_t>(row) * ld_src;
     __bf16* o = dst + static_cast<uint64_t>(row) * ld_dst;
-    double n2 = 0.0, n4 = 0.0, h2 = 0.0, h4 = 0.0;
+    double n2 = 0.0, n4 = 0.0, h2 = 0.0, h4 = 0.0, e4 = 0.0;
     for (uint32_t c = threadIdx.x; c < ld_dst; c += blockDim.x) {
         const float x = c < dim ? s[c] : 0.0F;
         const __bf16 h = static_cast<__bf16>(x);
         o[c] = h;
         const double xd = x;
         const double hd = static_cast<float>(h);
+        const double ed = xd - hd;
         n2 += xd * xd;
         n4 += (xd * xd) * (xd * xd);
         h2 += hd * hd;
         h4 += (hd * hd) * (hd * hd);
+        e4 += (ed * ed) * (ed * ed);
     }
     n2 = block_sum_f64(n2, scratch);
     n4 = block_sum_f64(n4, scratch);
     h2 = block_sum_f64(h2, scratch);
     h4 = block_sum_f64(h4, scratch);
+    e4 = block_sum_f64(e4, scratch);
     if (threadIdx.x == 0) {
-        RowStats st;
+        RowStats st{};
         st.norm2 = static_cast<float>(n2);
-        st.l4 = static_cast<float>(sqrt(sqrt(n4)));
-        st.l4_hat = static_cast<float>(sqrt(sqrt(h4)));
-        st.l2_hat = static_cast<float>(sqrt(h2));
+        // Rounded up a little so the stored bounds never understate.
+        st.l4 = static_cast<float>(sqrt(sqrt(n4))) * 1.0001F;
+        st.l4_hat = static_cast<float>(sqrt(sqrt(h4))) * 1.0001F;
+        st.l2_hat = static_cast<float>(sqrt(h2)) * 1.0001F;
+        st.d4 = static_cast<float>(sqrt(sqrt(e4))) * 1.0001F;
         stats[row] = st;
     }
 }
@@ -250,10 +255,16 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
 #undef UHANN_GRAM_STORE
 #undef UHANN_GRAM_COMPUTE
 
+// <q,v> - <q^,v^> = sum_i (q_i dv_i + v^_i dq_i) with dv = v - v^, dq = q - q^.
+// Modelling the rounding signs as independent and symmetric, Hoeffding with
+// c_i = |q_i dv_i| + |v^_i dq_i| and Minkowski + Cauchy-Schwarz
+// (sum c_i^2 <= (||q||_4 ||dv||_4 + ||v^||_4 ||dq||_4)^2) bound the error by
+// K (||q||_4 ||dv||_4 + ||v^||_4 ||dq||_4) except with probability
+// 2 exp(-K^2/2); dv and dq are the rows' actual rounding errors.
 __device__ __forceinline__ float pair_tau(const RowStats& q, const RowStats& v, float acc_steps_sqrt)
 {
-    const float sv = sqrtf(v.l4 * v.l4 + v.l4_hat * v.l4_hat);
-    const float dot_bound = kHoeffdingK * (kUbf16 * q.l4 * sv + kU32 * acc_steps_sqrt * q.l2_hat * v.l2_hat);
+    const float dot_bound = kHoeffdingK * (q.l4 * v.d4 + v.l4_hat * q.d4 +
+                                           kU32 * acc_steps_sqrt * q.l2_hat * v.l2_hat);
     return 2.0F * dot_bound + 4.0F * kU32 * (q.norm2 + v.norm2);
 }
 

@@ -14,6 +14,8 @@ struct RowStats {
     float l4;      // ||x||_4
     float l4_hat;  // ||bf16(x)||_4
     float l2_hat;  // ||bf16(x)||_2
+    float d4;      // ||x - bf16(x)||_4, the row's actual rounding error
+    float pad[3];
 };
 
 struct MixedStats {
