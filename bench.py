@@ -122,6 +122,90 @@ def quality(torch, refs, queries, got_idx, exact_idx, l1):
     }
 
 
+def serving(uh, torch, refs_np, queries, args, n_indexes=64):
+    """Served workload: n_indexes resident flat / hierarchical, L1 / L2
+    indexes with r log-uniform in [33, r_all], T log-uniform in [4, 256],
+    p in {16, 32, 64}; requests of 1-256 device-resident queries for a random
+    index. A burst of requests arriving together measures the server's
+    capacity; an open-loop Poisson stream at 70% of it gives latency. The
+    mixed run swaps every hierarchical L2 index for its bf16-projection twin."""
+    r_all, d = refs_np.shape
+    B = queries.shape[0]
+    g = np.random.default_rng(args.seed + 5)
+    rows = np.exp(g.uniform(np.log(33), np.log(r_all), n_indexes)).astype(np.int64)
+    reps = np.exp(g.uniform(np.log(4), np.log(256), n_indexes)).astype(np.int64)
+    kinds = g.integers(0, 2, n_indexes)  # 0 flat, 1 hierarchical
+    metrics = g.integers(0, 2, n_indexes)
+    pdims = g.choice([16, 32, 64], n_indexes)
+    starts = [int(g.integers(0, r_all - r + 1)) for r in rows]
+    srv = uh.Server(hbm_budget_bytes=64 << 30, max_tick_queries=8192, max_tick_requests=512,
+                    scratch_floats=64 << 20, staging_floats=1 << 20, use_graph=True)
+    t0 = time.perf_counter()
+    ids32, ids_mixed, ks = [], [], []
+    for i in range(n_indexes):
+        sub = refs_np[starts[i]:starts[i] + rows[i]]
+        m = int(metrics[i])
+        probs = uh.importance_probabilities_device(m, sub)
+        seed = args.seed + 100 + i
+        if kinds[i] == 0:
+            sid = srv.register_flat(sub, probs, int(reps[i]), uh.Rng(seed), m)
+            ids32.append(sid)
+            ids_mixed.append(sid)
+        else:
+            sid = srv.register_hierarchical(sub, probs, int(reps[i]), int(pdims[i]), uh.Rng(seed), m)
+            ids32.append(sid)
+            if m == uh.METRIC_L2:
+                ids_mixed.append(srv.register_hierarchical(sub, probs, int(reps[i]), int(pdims[i]), uh.Rng(seed), m,
+                                                           uh.STRATEGY_PROJECTION_BF16))
+            else:
+                ids_mixed.append(sid)
+        ks.append(len(uh.build_coordinate_sample(probs, int(reps[i]), uh.Rng(seed))))
+    setup_s = time.perf_counter() - t0
+    log(f"serving: {n_indexes} indexes built in {setup_s:.1f} s")
+    tq = srv.autotune(queries[:32], 16)
+    torch.cuda.synchronize()
+
+    def trace(ids, n, rate):
+        arrivals = np.zeros(n) if rate is None else np.cumsum(g.exponential(1.0 / rate, n))
+        counts = g.integers(1, 257, n)
+        which = g.integers(0, n_indexes, n)
+        offs = [int(g.integers(0, B - c + 1)) for c in counts]
+        views = [queries[o:o + int(c)] for o, c in zip(offs, counts)]
+        return arrivals, [ids[w] for w in which], views, counts
+
+    def run(ids, n, rate):
+        srv.reset_stats()
+        reqs = srv.run_trace(*trace(ids, n, rate))
+        for r in reqs:
+            r.wait(timeout=120)
+        st = srv.stats()
+        return {"requests": st["requests"], "queries": st["queries"],
+                "queries_per_s": round(st["queries_per_s"], 1),
+                "requests_per_s": round(st["requests"] / st["wall_s"], 1) if st["wall_s"] > 0 else None,
+                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                "mean_tick_queries": round(st["mean_tick_queries"], 1), "ticks": st["ticks"],
+                "gpu_busy_frac": round(st["gpu_busy_s"] / st["wall_s"], 3) if st["wall_s"] > 0 else None}
+
+    run(ids32, 200, None)  # warm-up: graph captures per tick shape
+    out = {"indexes": n_indexes, "rows_range": [int(rows.min()), int(rows.max())],
+           "T_range": [int(reps.min()), int(reps.max())],
+           "unique_coordinates_range": [int(min(ks)), int(max(ks))],
+           "kinds": {"flat_l2": int(np.sum((kinds == 0) & (metrics == 0))),
+                     "flat_l1": int(np.sum((kinds == 0) & (metrics == 1))),
+                     "hier_l2": int(np.sum((kinds == 1) & (metrics == 0))),
+                     "hier_l1": int(np.sum((kinds == 1) & (metrics == 1)))},
+           "resident_bytes": srv.stats()["resident_bytes"], "setup_s": round(setup_s, 1),
+           "pairs_tq_autotuned": tq, "queries_on_device": True}
+    for label, ids in (("fp32", ids32), ("mixed", ids_mixed)):
+        burst = run(ids, 3000, None)
+        rate = 0.7 * burst["requests_per_s"]
+        n = int(min(20000, max(500, rate * 2.0)))
+        out[label] = {"burst": burst, "poisson_70pct": dict(run(ids, n, rate), offered_requests_per_s=round(rate, 1))}
+        log(f"serving {label}: {out[label]}")
+    srv.close()
+    return out
+
+
 def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
     """Secondary workloads at the same shape; each entry is independent."""
     r, d = refs_np.shape
@@ -208,9 +292,21 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
                    "queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3)}
             if exact_idx_l2 is not None and not l1:
                 row.update(quality(torch, refs, queries, got, exact_idx_l2, False))
+            if not l1:
+                # projection GEMM in bf16 on MFMA (binary32 accumulation)
+                wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od,
+                                                                  strategy=uh.STRATEGY_PROJECTION_BF16),
+                                 steps, 1, torch)
+                g16 = oi.cpu().numpy().astype(np.uint64)
+                bf = {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
+                      "same_as_fp32": round(float(np.mean(g16 == got)), 5)}
+                if exact_idx_l2 is not None:
+                    bf.update(quality(torch, refs, queries, g16, exact_idx_l2, False))
+                row["bf16_projection"] = bf
             rows[f"T32_p{p}"] = row
         return rows
 
+    add("serving_64_indexes", lambda: serving(uh, torch, refs_np, queries, args))
     add("exact_l1_fp32_direct", exact_l1)
     add("exact_l2_streaming", streaming)
     add("flat_l2", lambda: flat(uh.METRIC_L2))

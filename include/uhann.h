@@ -50,6 +50,11 @@ extern "C" {
  * a binary64 re-rank of binary32 near ties. New (the reference's GEMM
  * strategy, cuda_dense_l2_scan_index.hpp:21-22, is unrefined FP32 SGEMM). */
 #define UHANN_STRATEGY_GRAM_BF16 1
+/* Hierarchical L2 only. The |I| x p query projection runs as a GEMM in BF16
+ * on MFMA (FP32 accumulation) instead of the reference's binary32 K7 order
+ * (cuda_float_projection.cu:234-263); the scan over the projected vectors is
+ * the direct binary32 scan. New: the reference projects in binary32 only. */
+#define UHANN_STRATEGY_PROJECTION_BF16 2
 
 /* Index kinds reported by uhann_index_info. */
 #define UHANN_KIND_EXACT 0
@@ -301,6 +306,127 @@ int uhann_scan_autotune(uhann_scan* scan, uhann_workspace* ws, const float* devi
  * gather and projection run first). INVALID_ARGUMENT for hierarchical L1. */
 int uhann_index_autotune(uhann_index* index, uhann_workspace* ws, const float* device_queries,
                          uint64_t ld_queries, uint64_t max_batch, int* table, int capacity, int* count);
+
+/* ------------------------------------------------------------------------
+ * Serving loop. No counterpart in the reference, which runs one facade
+ * query_batch at a time (nearest_neighbor_benchmark.cpp:386-413 times them);
+ * each request here executes what that facade's query_batch computes
+ * (cuda_sampled_coordinate_l{1,2}_ann_index.cpp, cuda_hierarchical_l{1,2}_
+ * ann_index.cpp, cuda_exact_l{1,2}_index.cpp), bit for bit.
+ *
+ * A server owns a set of indexes on one device. Registered indexes are built
+ * in HBM when their device bytes fit the server's budget (first come, first
+ * served); the others wait, and removing an index admits them. Requests (one
+ * batch of queries for one index) are queued and a worker thread packs the
+ * queued requests of resident indexes into ticks: up to max_tick_queries
+ * query rows and max_tick_requests requests, whose work runs together in one
+ * set of grouped persistent kernels (one hipGraph per tick shape, captured on
+ * first use and replayed). Two ticks are in flight: requests that arrive
+ * while one runs are packed into the next as rows free up. A request's
+ * results do not depend on what else shares its tick.
+ * ------------------------------------------------------------------------ */
+typedef struct uhann_server uhann_server;
+
+typedef struct uhann_server_config {
+    int device;
+    uint64_t hbm_budget_bytes;   /* device bytes of resident indexes */
+    uint32_t max_tick_queries;   /* query rows per tick (>= the largest request) */
+    uint32_t max_tick_requests;  /* requests per tick */
+    uint64_t scratch_floats;     /* per tick: gathered + projected queries */
+    uint64_t staging_floats;     /* per tick: host queries copied to the device */
+    int use_graph;               /* 1: replay captured graphs, 0: launch the kernels */
+    int pairs_tq;                /* queries per block of the pair-per-thread scans (1, 2, 4) */
+} uhann_server_config;
+
+/* Index description for uhann_server_register. host_refs is borrowed until
+ * the index is resident (uhann_server_index_state); sample and projection are
+ * copied. kind EXACT ignores the sample; FLAT ignores the projection;
+ * HIERARCHICAL takes projection_dim x sample_count binary64 values drawn as
+ * uhann_build_projection draws them. strategy: DIRECT, or PROJECTION_BF16 for
+ * a hierarchical L2 index. */
+typedef struct uhann_index_spec {
+    int kind;
+    int metric;
+    int strategy;
+    const float* host_refs;
+    uint64_t rows;
+    uint64_t dim;
+    const uhann_sampled_column* sample;
+    uint64_t sample_count;
+    const double* projection;
+    uint64_t projection_dim;
+} uhann_index_spec;
+
+#define UHANN_INDEX_PENDING 0   /* waiting for HBM budget */
+#define UHANN_INDEX_RESIDENT 1
+#define UHANN_INDEX_FAILED 2    /* the build failed; requests report its error */
+
+typedef struct uhann_serving_stats {
+    uint64_t requests;           /* completed since the last reset */
+    uint64_t queries;
+    uint64_t ticks;
+    double p50_ms;               /* request latency: submit -> results ready */
+    double p99_ms;
+    double mean_ms;
+    double max_ms;
+    double wall_s;               /* first submit -> last completion */
+    double queries_per_s;        /* queries / wall_s */
+    double gpu_busy_s;           /* summed tick durations on the device */
+    double mean_tick_queries;
+    uint64_t resident_indexes;
+    uint64_t pending_indexes;
+    uint64_t resident_bytes;
+} uhann_serving_stats;
+
+/* One request of a replayed trace: submitted arrival_s seconds after the
+ * replay starts. */
+typedef struct uhann_trace_entry {
+    double arrival_s;
+    uint64_t index_id;
+    const float* queries;
+    uint64_t ld_queries;
+    uint64_t count;
+    int on_device;
+    uint64_t* out_idx;
+    float* out_dist;
+} uhann_trace_entry;
+
+int uhann_server_create(const uhann_server_config* config, uhann_server** out);
+/* Waits for queued requests, then frees every index and buffer. */
+void uhann_server_destroy(uhann_server* server);
+int uhann_server_register(uhann_server* server, const uhann_index_spec* spec, uint64_t* index_id);
+int uhann_server_index_state(uhann_server* server, uint64_t index_id, int* state, uint64_t* device_bytes);
+/* Waits until the index has no request in flight, frees it and admits waiting
+ * indexes that now fit. */
+int uhann_server_remove(uhann_server* server, uint64_t index_id);
+/* Queues count queries (row stride ld_queries floats >= the index dimension;
+ * host memory, or device memory on the server's device when on_device) for
+ * an index. queries, out_idx (count values) and out_dist (count values, or
+ * null) must stay valid until the request completes. Data errors (a
+ * non-finite query value the index reads, a non-finite projection or median)
+ * are reported by uhann_server_wait with the index's error type. */
+int uhann_server_submit(uhann_server* server, uint64_t index_id, const float* queries, int on_device,
+                        uint64_t ld_queries, uint64_t count, uint64_t* out_idx, float* out_dist,
+                        uint64_t* request_id);
+/* Waits up to timeout_s seconds (< 0: no limit). *done = 1 once the request
+ * completed; then *request_status is its status (uhann_last_error holds its
+ * message) and the id is released. */
+int uhann_server_wait(uhann_server* server, uint64_t request_id, double timeout_s, int* done,
+                      int* request_status);
+/* Submits the entries at their arrival times from the calling thread (open
+ * loop) and writes their request ids; returns when the last one is queued. */
+int uhann_server_run_trace(uhann_server* server, const uhann_trace_entry* entries, uint64_t count,
+                           uint64_t* request_ids);
+/* Waits until no request is queued or in flight. */
+int uhann_server_drain(uhann_server* server);
+int uhann_server_stats(uhann_server* server, uhann_serving_stats* out);
+int uhann_server_reset_stats(uhann_server* server);
+/* Times the pair-scan query tiles (1, 2, 4 queries per block) on one tick of
+ * `rows_per_index` device queries (row stride ld_queries, at least the widest
+ * resident index) for every resident index, keeps the fastest and writes it
+ * to *pairs_tq. The server must be idle. */
+int uhann_server_autotune(uhann_server* server, const float* device_queries, uint64_t ld_queries,
+                          uint32_t rows_per_index, int* pairs_tq);
 
 #ifdef __cplusplus
 }

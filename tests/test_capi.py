@@ -71,3 +71,35 @@ def test_null_handles_are_rejected():
     assert lib.uhann_scan_info(None, ctypes.byref(out), None, None, None) == 1
     assert lib.uhann_workspace_info(None, ctypes.byref(out), None) == 1
     assert b"null" in lib.uhann_last_error()
+
+
+def test_struct_layouts_match_ctypes(tmp_path):
+    """The ctypes mirrors of the ABI's structs have the C compiler's sizes and offsets."""
+    from ultrahigh_ann_amd import _native
+
+    structs = {
+        "uhann_server_config": _native.ServerConfig,
+        "uhann_index_spec": _native.IndexSpec,
+        "uhann_serving_stats": _native.ServingStats,
+        "uhann_trace_entry": _native.TraceEntry,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "uhann.h"', "int main(void){"]
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    assert os.system(f"gcc -std=c99 -I{ROOT}/include {src} -o {exe}") == 0
+    out = os.popen(str(exe)).read().split("\n")
+    got = {}
+    for line in out:
+        if line:
+            name, field, value = line.split()
+            got[(name, field)] = int(value)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)

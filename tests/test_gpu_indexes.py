@@ -221,6 +221,67 @@ def test_hierarchical_parity_bit_exact(uh, oracle_mod, metric_l1, r, d, b, T, p)
     np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
 
 
+def _bf16(a):
+    """binary32 -> bf16 (round to nearest even), returned as binary64."""
+    b = np.ascontiguousarray(a, F32).view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return b.astype(np.uint32).view(F32).astype(np.float64)
+
+
+def test_bf16_rounding_helper():
+    # the helper itself: ties go to even, 1 + 2^-8 is a tie between 1 and 1 + 2^-7
+    x = np.array([1 + 2.0**-8, 1 + 3 * 2.0**-8, 1 + 2.0**-9, -(1 + 2.0**-8), 3.4e38], F32)
+    np.testing.assert_array_equal(_bf16(x), [1.0, 1 + 2.0**-6, 1.0, -1.0, np.inf])
+
+
+@pytest.mark.parametrize("r,d,b,T,p", [(300, 4000, 70, 32, 64), (90, 1500, 33, 32, 16), (40, 700, 9, 8, 33)])
+def test_hierarchical_l2_bf16_projection(uh, oracle_mod, r, d, b, T, p):
+    """PROJECTION_BF16 = exact projection of the bf16-rounded operands up to
+    binary32 accumulation: distances within the derived bound, argmin equal
+    outside that band."""
+    refs, q = _random_index_case(r * 5 + p, r, d, b)
+    probs = oracle_mod.importance_probabilities(False, refs)
+    index = uh.HipHierarchicalL2AnnIndex(refs, probs, T, p, uh.Rng(23))
+    ws = index.make_query_workspace(b)
+    idx, dist = index.query_batch(q, b, ws, uh.STRATEGY_PROJECTION_BF16)
+    built = oracle_mod.build_hierarchical(False, refs, probs, T, p, 23)
+    k = built["cols"].shape[0]
+    xb = _bf16(_emb(q, built["cols"]))
+    pb = _bf16(built["p32"])
+    y = xb @ pb.T
+    # |y_gpu - y| <= (ceil(k/16) + 8) u32 sum_j |x_j P_j| (products are exact in binary32)
+    eps = ((k + 15) // 16 + 8) * 2.0**-24 * (np.abs(xb) @ np.abs(pb).T)
+    pr = built["proj_refs32"].astype(np.float64)
+    diff = y[:, None, :] - pr[None, :, :]
+    d64 = (diff**2).sum(axis=2)
+    tol = 2 * (np.abs(diff) * eps[:, None, :]).sum(axis=2) + (eps**2).sum(axis=1)[:, None] + (p + 10) * 2.0**-24 * d64
+    rows = np.arange(b)
+    chosen = idx.astype(np.int64)
+    assert np.all(np.abs(dist.astype(np.float64) - d64[rows, chosen]) <= tol[rows, chosen])
+    best = d64.argmin(axis=1)
+    near = d64[rows, chosen] - d64[rows, best] <= tol[rows, chosen] + tol[rows, best]
+    assert np.all((chosen == best) | near)
+    assert np.mean(chosen == best) > 0.9
+    # the binary32 path is unchanged by the bf16 copy
+    idx32, _ = index.query_batch(q, b, ws)
+    _, (e_idx, _) = _hier_expect(oracle_mod, False, refs, q, probs, T, p, 23)
+    np.testing.assert_array_equal(idx32, e_idx)
+
+
+def test_bf16_projection_wide_identity_and_l1_rejects(uh):
+    rows, d = 4, 513
+    col = np.arange(d)
+    refs = np.array([[2.0 * r + 0.01 * ((c % 7) - 3) for c in col] for r in range(rows)], F32)
+    sample = uh.make_sample(col, np.ones(d), np.ones(d))
+    index = uh.HipHierarchicalL2AnnIndex(refs, None, 0, 31, uh.Rng(1729), sample=sample)
+    ws = index.make_query_workspace(rows)
+    idx, _ = index.query_batch(refs, rows, ws, uh.STRATEGY_PROJECTION_BF16)
+    assert list(idx) == list(range(rows))
+    l1 = uh.HipHierarchicalL1AnnIndex(refs, None, 0, 17, uh.Rng(1729), sample=sample)
+    with pytest.raises(uh.UhannInvalidArgument):
+        l1.query_batch(refs, rows, l1.make_query_workspace(rows), uh.STRATEGY_PROJECTION_BF16)
+
+
 def test_hierarchical_single_row_and_errors(uh):
     refs = np.array([[1.0, 2.0, 3.0]], F32)
     index = uh.HipHierarchicalL2AnnIndex(refs, np.array([1.0, 1.0, 1.0]), 2, 3, uh.Rng(3))

@@ -51,17 +51,20 @@ from .indexes import (  # noqa: E402
     importance_probabilities_device,
     make_sample,
 )
+from .serving import INDEX_FAILED, INDEX_PENDING, INDEX_RESIDENT, Server  # noqa: E402
 
 METRIC_L2 = 0
 METRIC_L1 = 1
 STRATEGY_DIRECT = 0
 STRATEGY_GRAM_BF16 = 1
+STRATEGY_PROJECTION_BF16 = 2
 
 __all__ = [
     "METRIC_L1",
     "METRIC_L2",
     "STRATEGY_DIRECT",
     "STRATEGY_GRAM_BF16",
+    "STRATEGY_PROJECTION_BF16",
     "HipDenseL1ScanIndex",
     "HipExactIndex",
     "HipFlatL1AnnIndex",
@@ -76,6 +79,10 @@ __all__ = [
     "importance_probabilities",
     "importance_probabilities_device",
     "make_sample",
+    "Server",
+    "INDEX_FAILED",
+    "INDEX_PENDING",
+    "INDEX_RESIDENT",
     "HipDenseL2ScanIndex",
     "HipExactL1Index",
     "HipExactL2Index",

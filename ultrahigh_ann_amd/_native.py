@@ -57,6 +57,38 @@ _STATUS = {
     7: UhannLogicError,
 }
 
+class ServerConfig(ctypes.Structure):
+    """uhann_server_config."""
+
+    _fields_ = [("device", c_int), ("hbm_budget_bytes", c_uint64), ("max_tick_queries", ctypes.c_uint32),
+                ("max_tick_requests", ctypes.c_uint32), ("scratch_floats", c_uint64), ("staging_floats", c_uint64),
+                ("use_graph", c_int), ("pairs_tq", c_int)]
+
+
+class IndexSpec(ctypes.Structure):
+    """uhann_index_spec."""
+
+    _fields_ = [("kind", c_int), ("metric", c_int), ("strategy", c_int), ("host_refs", c_void_p),
+                ("rows", c_uint64), ("dim", c_uint64), ("sample", c_void_p), ("sample_count", c_uint64),
+                ("projection", c_void_p), ("projection_dim", c_uint64)]
+
+
+class ServingStats(ctypes.Structure):
+    """uhann_serving_stats."""
+
+    _fields_ = [("requests", c_uint64), ("queries", c_uint64), ("ticks", c_uint64), ("p50_ms", c_double),
+                ("p99_ms", c_double), ("mean_ms", c_double), ("max_ms", c_double), ("wall_s", c_double),
+                ("queries_per_s", c_double), ("gpu_busy_s", c_double), ("mean_tick_queries", c_double),
+                ("resident_indexes", c_uint64), ("pending_indexes", c_uint64), ("resident_bytes", c_uint64)]
+
+
+class TraceEntry(ctypes.Structure):
+    """uhann_trace_entry."""
+
+    _fields_ = [("arrival_s", c_double), ("index_id", c_uint64), ("queries", c_void_p), ("ld_queries", c_uint64),
+                ("count", c_uint64), ("on_device", c_int), ("out_idx", c_void_p), ("out_dist", c_void_p)]
+
+
 _lib = None
 
 
@@ -118,6 +150,19 @@ def _declare(lib):
                                         POINTER(c_int)]),
         "uhann_index_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, POINTER(c_int), c_int,
                                          POINTER(c_int)]),
+        "uhann_server_create": (c_int, [POINTER(ServerConfig), POINTER(c_void_p)]),
+        "uhann_server_destroy": (None, [c_void_p]),
+        "uhann_server_register": (c_int, [c_void_p, POINTER(IndexSpec), u64p]),
+        "uhann_server_index_state": (c_int, [c_void_p, c_uint64, POINTER(c_int), u64p]),
+        "uhann_server_remove": (c_int, [c_void_p, c_uint64]),
+        "uhann_server_submit": (c_int, [c_void_p, c_uint64, c_void_p, c_int, c_uint64, c_uint64, c_void_p, c_void_p,
+                                        u64p]),
+        "uhann_server_wait": (c_int, [c_void_p, c_uint64, c_double, POINTER(c_int), POINTER(c_int)]),
+        "uhann_server_run_trace": (c_int, [c_void_p, POINTER(TraceEntry), c_uint64, u64p]),
+        "uhann_server_drain": (c_int, [c_void_p]),
+        "uhann_server_stats": (c_int, [c_void_p, POINTER(ServingStats)]),
+        "uhann_server_reset_stats": (c_int, [c_void_p]),
+        "uhann_server_autotune": (c_int, [c_void_p, c_void_p, c_uint64, ctypes.c_uint32, POINTER(c_int)]),
     }
     optional = {}
     for name, (res, args) in sigs.items():

@@ -1,0 +1,524 @@
+// Device bodies of the scan, gather, projection and median kernels. Each
+// body computes one tile given its coordinates; the one-index kernels in
+// scan.hip, gather.hip and hier.hip pass their block coordinates, and the
+// grouped serving kernels in serve.hip pass the coordinates of a work unit
+// of one request, so both run the same arithmetic in the same order.
+// Bodies use the whole block (barriers inside); callers that loop over
+// several tiles put a barrier between two calls.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace uhann::kern::body {
+
+constexpr int kChains = 256;
+constexpr int kGroups = 8;  // groups of 32 chains
+
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int RPT>
+__device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&v)[RPT])
+{
+    if constexpr (RPT == 1) {
+        v[0] = p[0];
+    } else if constexpr (RPT == 2) {
+        const float2 x = *reinterpret_cast<const float2*>(p);
+        v[0] = x.x;
+        v[1] = x.y;
+    } else {
+        static_assert(RPT == 4, "RPT must be 1, 2 or 4");
+        const float4 x = *reinterpret_cast<const float4*>(p);
+        v[0] = x.x;
+        v[1] = x.y;
+        v[2] = x.z;
+        v[3] = x.w;
+    }
+}
+
+template <int METRIC, bool WEIGHTED, bool WFMA>
+__device__ __forceinline__ float accumulate(float acc, float r, float q, float w)
+{
+    const float d = r - q;
+    if constexpr (METRIC == 0) {
+        return __builtin_fmaf(d, d, acc);
+    } else {
+        const float ad = __builtin_fabsf(d);
+        if constexpr (!WEIGHTED) {
+            return acc + ad;
+        } else if constexpr (WFMA) {
+            return __builtin_fmaf(w, ad, acc);
+        } else {
+            const float prod = w * ad;
+            return acc + prod;
+        }
+    }
+}
+
+// One TQ x TR tile of the direct scan (queries q0.., references r0..).
+template <int METRIC, bool WEIGHTED, bool WFMA, int RPT, int TQ, int TR>
+__device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32_t r0)
+{
+    constexpr int NT = kChains / RPT;     // threads per block
+    constexpr int GROUP = 32 / RPT;       // threads per group of 32 chains
+    constexpr int NP = TQ * TR;           // pairs per block
+    const int t = threadIdx.x;
+    const float* rrow[TR];
+    const float* qrow[TQ];
+#pragma unroll
+    for (int j = 0; j < TR; ++j) {
+        rrow[j] = a.refs + static_cast<uint64_t>(min(r0 + j, a.rows - 1U)) * a.ld_ref;
+    }
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) {
+        qrow[i] = a.queries + static_cast<uint64_t>(min(q0 + i, a.nq - 1U)) * a.ld_q;
+    }
+
+    float acc[TQ][TR][RPT];
+#pragma unroll
+    for (int i = 0; i < TQ; ++i)
+#pragma unroll
+        for (int j = 0; j < TR; ++j)
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) acc[i][j][k] = 0.0F;
+
+    // One 256-column step at column `col` (this thread's first chain). In the
+    // tail step a vector may straddle dim (rows are padded to a multiple of 4
+    // floats): components at or past dim are skipped, so pad values are
+    // never added.
+    auto step = [&](uint32_t col, bool masked) {
+        float rv[TR][RPT];
+        float qv[TQ][RPT];
+        float wv[RPT];
+#pragma unroll
+        for (int j = 0; j < TR; ++j) load_vec<RPT>(rrow[j] + col, rv[j]);
+#pragma unroll
+        for (int i = 0; i < TQ; ++i) load_vec<RPT>(qrow[i] + col, qv[i]);
+        if constexpr (WEIGHTED) {
+            load_vec<RPT>(a.weights + col, wv);
+        } else {
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) wv[k] = 0.0F;
+        }
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (masked && col + static_cast<uint32_t>(k) >= a.dim) continue;
+#pragma unroll
+            for (int i = 0; i < TQ; ++i)
+#pragma unroll
+                for (int j = 0; j < TR; ++j)
+                    acc[i][j][k] = accumulate<METRIC, WEIGHTED, WFMA>(acc[i][j][k], rv[j][k],
+                                                                     qv[i][k], wv[k]);
+        }
+    };
+
+    const uint32_t col_t = static_cast<uint32_t>(RPT * t);
+    const uint32_t full_end = (a.dim / kChains) * kChains;
+#pragma unroll 2
+    for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false);
+    if (full_end + col_t < a.dim) step(full_end + col_t, true);
+
+    // Replay the reference tree. Cross-thread steps first (chain offsets 16..RPT
+    // are thread offsets GROUP/2..1), then the chains inside the thread.
+    __shared__ float group_sums[kGroups][NP];
+    __shared__ float pair_dist[NP];
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) {
+#pragma unroll
+        for (int j = 0; j < TR; ++j) {
+            float v[RPT];
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) v[k] = acc[i][j][k];
+#pragma unroll
+            for (int off = GROUP / 2; off >= 1; off /= 2) {
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) v[k] = v[k] + __shfl_down(v[k], off);
+            }
+            float w;
+            if constexpr (RPT == 1) {
+                w = v[0];
+            } else if constexpr (RPT == 2) {
+                w = v[0] + v[1];
+            } else {
+                w = (v[0] + v[2]) + (v[1] + v[3]);
+            }
+            if (t % GROUP == 0) group_sums[t / GROUP][i * TR + j] = w;
+        }
+    }
+    __syncthreads();
+    for (int p = t; p < NP; p += NT) {
+        const float g0 = group_sums[0][p], g1 = group_sums[1][p];
+        const float g2 = group_sums[2][p], g3 = group_sums[3][p];
+        const float g4 = group_sums[4][p], g5 = group_sums[5][p];
+        const float g6 = group_sums[6][p], g7 = group_sums[7][p];
+        // Lanes 8..31 of the reference's second warp_sum hold +0.0f; adding
+        // them leaves a non-negative sum unchanged, so they are skipped.
+        pair_dist[p] = ((g0 + g4) + (g2 + g6)) + ((g1 + g5) + (g3 + g7));
+    }
+    __syncthreads();
+    if (t < TQ) {
+        const uint32_t q = q0 + static_cast<uint32_t>(t);
+        if (q < a.nq) {
+            unsigned long long best = ~0ULL;
+#pragma unroll
+            for (int j = 0; j < TR; ++j) {
+                const uint32_t r = r0 + static_cast<uint32_t>(j);
+                if (r < a.rows) {
+                    const unsigned long long key =
+                        (static_cast<unsigned long long>(__float_as_uint(pair_dist[t * TR + j]))
+                         << 32) |
+                        r;
+                    best = key < best ? key : best;
+                }
+            }
+            atomicMin(a.keys + q, best);
+        }
+    }
+}
+
+
+// Pair-per-thread form for short rows (flat |I|, hierarchical p): the
+// 256-chain tree costs 255 adds per pair whatever the row length, and in the
+// tiled kernel above it is a cross-lane tree, which dominates when a chain
+// holds one or two terms. Here each thread owns one reference row and TQ
+// queries and evaluates the whole reference order in registers: for each
+// group w of 32 chains, chain 32w+i sums columns 32w+i, 32w+i+256, ... in
+// order, then the 32-lane tree (offsets 16..1) gives W_w, and
+// ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7)) the distance -- the same additions as
+// the reference, so the same bits. References are read from a transposed
+// (dim x rows) copy, so a wave's loads are contiguous; queries (and L1
+// weights) sit in LDS and are broadcast.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__device__ __forceinline__ void scan_pairs(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
+                                           const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
+                                           const float* __restrict__ weights, uint32_t dim,
+                                           unsigned long long* __restrict__ keys, uint32_t q0, uint32_t rblock,
+                                           float* smem)
+{
+    float* qs = smem;              // TQ x dim
+    float* ws = smem + TQ * dim;   // dim (weighted L1)
+    const int t = threadIdx.x;
+    for (uint32_t e = t; e < TQ * dim; e += 256) {
+        const uint32_t qi = e / dim;
+        const uint32_t c = e % dim;
+        qs[e] = queries[static_cast<uint64_t>(min(q0 + qi, nq - 1U)) * ld_q + c];
+    }
+    if constexpr (WEIGHTED) {
+        for (uint32_t c = t; c < dim; c += 256) ws[c] = weights[c];
+    }
+    __syncthreads();
+    const uint32_t r = rblock * 256 + t;
+    const uint32_t rr = min(r, rows - 1U);
+    // Groups are visited in bit-reversed order 0,4 | 2,6 | 1,5 | 3,7, so pass
+    // p of 4 produces (W0+W4), (W2+W6), (W1+W5), (W3+W7) in turn and the
+    // distance is (P0 + P1) + (P2 + P3): the reference's final tree with only
+    // four partial sums live (the loops stay rolled to keep registers low).
+    float P[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[q][k] = 0.0F;
+#pragma unroll 1
+    for (int pass = 0; pass < 4; ++pass) {
+        float pair_sum[TQ];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const int g = 2 * pass + h;
+            const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
+            float v[TQ][32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) v[q][i] = 0.0F;
+            // Term k of chain 32w+i is column 32w + i + 256k: all 32 chains of
+            // the group load term k together (32 independent loads in flight),
+            // and each chain still adds its terms in increasing k.
+            for (uint32_t base = static_cast<uint32_t>(32 * w); base < dim; base += kChains) {
+                float rv[32];
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const uint32_t c = min(base + i, dim - 1U);
+                    rv[i] = refs_t[static_cast<uint64_t>(c) * ld_t + rr];
+                }
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const uint32_t c = base + i;
+                    if (c < dim) {
+                        const float wv = WEIGHTED ? ws[c] : 0.0F;
+#pragma unroll
+                        for (int q = 0; q < TQ; ++q) {
+                            v[q][i] = accumulate<METRIC, WEIGHTED, WFMA>(v[q][i], rv[i], qs[q * dim + c], wv);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) {
+#pragma unroll
+                for (int off = 16; off >= 1; off /= 2) {
+#pragma unroll
+                    for (int i = 0; i < off; ++i) v[q][i] = v[q][i] + v[q][i + off];
+                }
+                pair_sum[q] = h == 0 ? v[q][0] : pair_sum[q] + v[q][0];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) P[q][k] = pass == k ? pair_sum[q] : P[q][k];
+    }
+    __shared__ unsigned long long best[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+        const float dist = (P[q][0] + P[q][1]) + (P[q][2] + P[q][3]);
+        unsigned long long key = r < rows
+                                     ? (static_cast<unsigned long long>(__float_as_uint(dist)) << 32) | r
+                                     : ~0ULL;
+        for (int off = 32; off >= 1; off /= 2) {
+            const unsigned long long o = __shfl_down(key, off);
+            key = o < key ? o : key;
+        }
+        if (t % 64 == 0) best[q][t / 64] = key;
+    }
+    __syncthreads();
+    if (t < TQ && q0 + t < nq) {
+        unsigned long long b = best[t][0];
+        for (int w = 1; w < 4; ++w) b = best[t][w] < b ? best[t][w] : b;
+        atomicMin(keys + q0 + t, b);
+    }
+}
+
+// Gather tile: sampled-column chunk `chunk` (kGatherChunk columns) of query
+// rows row_block*kGatherRows ... 
+__device__ __forceinline__ void gather_tile(const float* __restrict__ src, uint64_t ld_src, uint32_t nq,
+                                            const uint32_t* __restrict__ cols, const float* __restrict__ scales,
+                                            uint32_t k, float* __restrict__ out, uint64_t ld_out, int* invalid,
+                                            uint32_t chunk, uint32_t row_block)
+{
+    __shared__ uint32_t s_cols[kGatherChunk];
+    __shared__ float s_scales[kGatherChunk];
+    const uint32_t c0 = chunk * kGatherChunk;
+    const uint32_t n = min(static_cast<uint32_t>(kGatherChunk), k - c0);
+    for (uint32_t i = threadIdx.x; i < n; i += kGatherThreads) {
+        s_cols[i] = cols[c0 + i];
+        s_scales[i] = scales != nullptr ? scales[c0 + i] : 1.0F;
+    }
+    __syncthreads();
+    bool bad = false;
+    const uint32_t q_begin = row_block * kGatherRows;
+    const uint32_t q_end = min(q_begin + kGatherRows, nq);
+    for (uint32_t q = q_begin; q < q_end; ++q) {
+        const float* row = src + static_cast<uint64_t>(q) * ld_src;
+        float* dst = out + static_cast<uint64_t>(q) * ld_out + c0;
+        for (uint32_t i = threadIdx.x; i < n; i += kGatherThreads) {
+            const float v = row[s_cols[i]];
+            const float scaled = scales != nullptr ? v * s_scales[i] : v;
+            bad |= !__builtin_isfinite(v) || !__builtin_isfinite(scaled);
+            dst[i] = scaled;
+        }
+    }
+    if (bad) atomicExch(invalid, 1);
+}
+
+// One kProjQ x kProjP tile of the binary32 projection (K7 order).
+__device__ __forceinline__ void project_fp32_tile(const float* __restrict__ x, uint64_t ld_x, uint32_t nq,
+                                                  uint32_t k, const float* __restrict__ proj, uint64_t ld_proj,
+                                                  uint32_t p, float* __restrict__ y, uint64_t ld_y, int* invalid,
+                                                  uint32_t q0, uint32_t o0)
+{
+    const int t = threadIdx.x;
+    const float* xr[kProjQ];
+    const float* pr[kProjP];
+#pragma unroll
+    for (int i = 0; i < kProjQ; ++i) xr[i] = x + static_cast<uint64_t>(min(q0 + i, nq - 1U)) * ld_x;
+#pragma unroll
+    for (int j = 0; j < kProjP; ++j) pr[j] = proj + static_cast<uint64_t>(min(o0 + j, p - 1U)) * ld_proj;
+    float acc[kProjQ][kProjP];
+#pragma unroll
+    for (int i = 0; i < kProjQ; ++i)
+#pragma unroll
+        for (int j = 0; j < kProjP; ++j) acc[i][j] = 0.0F;
+    for (uint32_t c = t; c < k; c += kProjThreads) {
+        float xv[kProjQ], pv[kProjP];
+#pragma unroll
+        for (int i = 0; i < kProjQ; ++i) xv[i] = xr[i][c];
+#pragma unroll
+        for (int j = 0; j < kProjP; ++j) pv[j] = pr[j][c];
+#pragma unroll
+        for (int i = 0; i < kProjQ; ++i)
+#pragma unroll
+            for (int j = 0; j < kProjP; ++j) acc[i][j] = __builtin_fmaf(xv[i], pv[j], acc[i][j]);
+    }
+    __shared__ float warp_sums[8][kProjQ * kProjP];
+#pragma unroll
+    for (int i = 0; i < kProjQ; ++i) {
+#pragma unroll
+        for (int j = 0; j < kProjP; ++j) {
+            float v = acc[i][j];
+            for (int off = 16; off >= 1; off /= 2) v = v + __shfl_down(v, off);
+            if (t % 32 == 0) warp_sums[t / 32][i * kProjP + j] = v;
+        }
+    }
+    __syncthreads();
+    if (t < kProjQ * kProjP) {
+        const int i = t / kProjP;
+        const int j = t % kProjP;
+        const uint32_t q = q0 + i;
+        const uint32_t o = o0 + j;
+        if (q < nq && o < p) {
+            float w[8];
+#pragma unroll
+            for (int g = 0; g < 8; ++g) w[g] = (warp_sums[g][t] + 0.0F) + 0.0F;
+            const float s = ((w[0] + w[4]) + (w[2] + w[6])) + ((w[1] + w[5]) + (w[3] + w[7]));
+            if (!__builtin_isfinite(s)) atomicExch(invalid, 1);
+            y[static_cast<uint64_t>(q) * ld_y + o] = s;
+        }
+    }
+}
+
+// One 32 x 32 tile of the bf16 MFMA projection.
+__device__ __forceinline__ void project_bf16_tile(const float* __restrict__ x, uint64_t ld_x, uint32_t nq,
+                                                  uint32_t k, const __bf16* __restrict__ proj, uint64_t ld_proj,
+                                                  uint32_t p, float* __restrict__ y, uint64_t ld_y, int* invalid,
+                                                  uint32_t q0, uint32_t o0)
+{
+    const int lane = threadIdx.x % 64;
+    const int wave = threadIdx.x / 64;
+    // MFMA operand fragments: lane holds row (lane % 32) and K values
+    // 8*(lane/32) .. +7 of each 16-wide K step. Rows of x are a multiple of
+    // four floats long, so a float4 that starts below k stays in the row.
+    const float* xr = x + static_cast<uint64_t>(min(q0 + lane % 32, nq - 1U)) * ld_x + 8 * (lane / 32);
+    const __bf16* pr = proj + static_cast<uint64_t>(o0 + lane % 32) * ld_proj + 8 * (lane / 32);
+    const uint32_t steps = (k + 15U) / 16U;
+    floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.0F;
+    for (uint32_t s = wave; s < steps; s += kProjWaves) {
+        const uint32_t c0 = s * 16U + 8U * (lane / 32);
+        float xv[8];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float4 f = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+            if (c0 + 4U * h < k) f = *reinterpret_cast<const float4*>(xr + s * 16U + 4U * h);
+            xv[4 * h + 0] = f.x;
+            xv[4 * h + 1] = f.y;
+            xv[4 * h + 2] = f.z;
+            xv[4 * h + 3] = f.w;
+        }
+        bf16x8 a;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] = static_cast<__bf16>(c0 + e < k ? xv[e] : 0.0F);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(pr + s * 16U);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __shared__ float part[kProjWaves][32 * 32];
+    // D layout: lane holds column (lane % 32) and rows 8*(v/4) + 4*(lane/32) + v%4.
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+        part[wave][row * 32 + lane % 32] = acc[v];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 32; e += 64 * kProjWaves) {
+        const uint32_t q = q0 + e / 32;
+        const uint32_t o = o0 + e % 32;
+        if (q < nq && o < p) {
+            const float s = (part[0][e] + part[1][e]) + (part[2][e] + part[3][e]);
+            if (!__builtin_isfinite(s)) atomicExch(invalid, 1);
+            y[static_cast<uint64_t>(q) * ld_y + o] = s;
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void bitonic_sort(float (&v)[N])
+{
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ stride;
+                if (l > i) {
+                    const float a = v[i];
+                    const float b = v[l];
+                    const float lo = fminf(a, b);
+                    const float hi = fmaxf(a, b);
+                    if ((i & size) == 0) {
+                        v[i] = lo;
+                        v[l] = hi;
+                    } else {
+                        v[i] = hi;
+                        v[l] = lo;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Median-L1 estimates of query q against references rblock*256 ... +255.
+template <int PMAX>
+__device__ __forceinline__ void median_block(const float* __restrict__ refs_t, uint32_t rows,
+                                             const float* __restrict__ yq, uint64_t ld_y, uint32_t nq, uint32_t p,
+                                             unsigned long long* __restrict__ keys, int* invalid, uint32_t q,
+                                             uint32_t rblock)
+{
+    __shared__ float qv[PMAX];
+    __shared__ unsigned long long best_key[kMedianThreads / 64];
+    if (q >= nq) return;
+    const int t = threadIdx.x;
+    for (uint32_t c = t; c < p; c += kMedianThreads) qv[c] = yq[static_cast<uint64_t>(q) * ld_y + c];
+    __syncthreads();
+    const uint32_t r = rblock * kMedianThreads + t;
+    unsigned long long key = ~0ULL;
+    if (r < rows) {
+        float v[PMAX];
+        bool bad = false;
+#pragma unroll
+        for (int c = 0; c < PMAX; ++c) {
+            if (static_cast<uint32_t>(c) < p) {
+                float d = __builtin_fabsf(refs_t[static_cast<uint64_t>(c) * rows + r] - qv[c]);
+                if (!__builtin_isfinite(d)) {
+                    bad = true;
+                    d = __builtin_inff();
+                }
+                v[c] = d;
+            } else {
+                v[c] = __builtin_inff();
+            }
+        }
+        bitonic_sort<PMAX>(v);
+        const uint32_t mid = p / 2;
+        float med = 0.0F;
+        float lower = 0.0F;
+#pragma unroll
+        for (int c = 0; c < PMAX; ++c) {
+            if (static_cast<uint32_t>(c) == mid) med = v[c];
+            if (static_cast<uint32_t>(c) + 1U == mid) lower = v[c];
+        }
+        if (p % 2 == 0) med = lower + (med - lower) * 0.5F;
+        if (!__builtin_isfinite(med)) bad = true;
+        if (bad) atomicExch(invalid, 1);
+        key = (static_cast<unsigned long long>(__float_as_uint(med)) << 32) | r;
+    }
+    for (int off = 32; off >= 1; off /= 2) {
+        const unsigned long long other = __shfl_down(key, off);
+        key = other < key ? other : key;
+    }
+    if (t % 64 == 0) best_key[t / 64] = key;
+    __syncthreads();
+    if (t == 0) {
+        unsigned long long b = best_key[0];
+        for (int w = 1; w < kMedianThreads / 64; ++w) b = best_key[w] < b ? best_key[w] : b;
+        atomicMin(keys + q, b);
+    }
+}
+
+}  // namespace uhann::kern::body

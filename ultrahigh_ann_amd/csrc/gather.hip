@@ -16,44 +16,23 @@
 #include <stdexcept>
 #include <string>
 
+#include "bodies.hpp"
 #include "gram.hpp"
 #include "kernels.hpp"
 
 namespace uhann::kern {
 namespace {
 
-constexpr int kGatherThreads = 256;
-constexpr int kGatherChunk = 1024;
-constexpr int kGatherRows = 8;  // query rows per block
+using body::kGatherChunk;
+using body::kGatherRows;
+using body::kGatherThreads;
 
 __global__ void __launch_bounds__(kGatherThreads)
 gather_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t nq,
               const uint32_t* __restrict__ cols, const float* __restrict__ scales, uint32_t k,
               float* __restrict__ out, uint64_t ld_out, int* invalid)
 {
-    __shared__ uint32_t s_cols[kGatherChunk];
-    __shared__ float s_scales[kGatherChunk];
-    const uint32_t c0 = blockIdx.x * kGatherChunk;
-    const uint32_t n = min(static_cast<uint32_t>(kGatherChunk), k - c0);
-    for (uint32_t i = threadIdx.x; i < n; i += kGatherThreads) {
-        s_cols[i] = cols[c0 + i];
-        s_scales[i] = scales != nullptr ? scales[c0 + i] : 1.0F;
-    }
-    __syncthreads();
-    bool bad = false;
-    const uint32_t q_begin = blockIdx.y * kGatherRows;
-    const uint32_t q_end = min(q_begin + kGatherRows, nq);
-    for (uint32_t q = q_begin; q < q_end; ++q) {
-        const float* row = src + static_cast<uint64_t>(q) * ld_src;
-        float* dst = out + static_cast<uint64_t>(q) * ld_out + c0;
-        for (uint32_t i = threadIdx.x; i < n; i += kGatherThreads) {
-            const float v = row[s_cols[i]];
-            const float scaled = scales != nullptr ? v * s_scales[i] : v;
-            bad |= !__builtin_isfinite(v) || !__builtin_isfinite(scaled);
-            dst[i] = scaled;
-        }
-    }
-    if (bad) atomicExch(invalid, 1);
+    body::gather_tile(src, ld_src, nq, cols, scales, k, out, ld_out, invalid, blockIdx.x, blockIdx.y);
 }
 
 __global__ void check_finite_kernel(const float* __restrict__ values, uint64_t n, int* invalid)

@@ -10,6 +10,14 @@
 // kProjQ x kProjP tile of outputs; each loaded x value is reused kProjP times
 // and each P value kProjQ times.
 //
+// project_bf16 is the MFMA form of the same projection for the L2 hierarchy:
+// y = x P^T with both operands rounded to bf16 (round to nearest even) and
+// v_mfma_f32_32x32x16_bf16 accumulating in binary32. A block of four waves
+// owns a 32 x 32 output tile; wave w takes the K steps w, w + 4, ... and the
+// four partial tiles are added in LDS in a fixed order, so the result is
+// deterministic (it is not the binary32 path's bits: the rounding of x and P
+// to bf16 changes the projection by about 2^-9 relative).
+//
 // median_l1 replaces K9/K10 (cuda_median_l1_scan_index.cu:202-301): the
 // estimate for a pair is the median of |r_k - q_k| over the p projected
 // coordinates (even p: lower + (upper - lower) * 0.5f). The median depends
@@ -25,14 +33,17 @@
 #include <stdexcept>
 #include <string>
 
+#include "bodies.hpp"
 #include "hier.hpp"
 
 namespace uhann::kern {
 namespace {
 
-constexpr int kProjThreads = 256;
-constexpr int kProjQ = 4;
-constexpr int kProjP = 8;
+using body::kMedianThreads;
+using body::kProjP;
+using body::kProjQ;
+using body::kProjThreads;
+using body::kProjWaves;
 
 void check_launch(const char* what)
 {
@@ -51,85 +62,16 @@ project_fp32_kernel(const float* __restrict__ x, uint64_t ld_x, uint32_t nq, uin
     const uint32_t tiles_p = (p + kProjP - 1) / kProjP;
     const uint32_t q0 = (blockIdx.x / tiles_p) * kProjQ;
     const uint32_t o0 = (blockIdx.x % tiles_p) * kProjP;
-    const int t = threadIdx.x;
-    const float* xr[kProjQ];
-    const float* pr[kProjP];
-#pragma unroll
-    for (int i = 0; i < kProjQ; ++i) xr[i] = x + static_cast<uint64_t>(min(q0 + i, nq - 1U)) * ld_x;
-#pragma unroll
-    for (int j = 0; j < kProjP; ++j) pr[j] = proj + static_cast<uint64_t>(min(o0 + j, p - 1U)) * ld_proj;
-    float acc[kProjQ][kProjP];
-#pragma unroll
-    for (int i = 0; i < kProjQ; ++i)
-#pragma unroll
-        for (int j = 0; j < kProjP; ++j) acc[i][j] = 0.0F;
-    for (uint32_t c = t; c < k; c += kProjThreads) {
-        float xv[kProjQ], pv[kProjP];
-#pragma unroll
-        for (int i = 0; i < kProjQ; ++i) xv[i] = xr[i][c];
-#pragma unroll
-        for (int j = 0; j < kProjP; ++j) pv[j] = pr[j][c];
-#pragma unroll
-        for (int i = 0; i < kProjQ; ++i)
-#pragma unroll
-            for (int j = 0; j < kProjP; ++j) acc[i][j] = __builtin_fmaf(xv[i], pv[j], acc[i][j]);
-    }
-    __shared__ float warp_sums[8][kProjQ * kProjP];
-#pragma unroll
-    for (int i = 0; i < kProjQ; ++i) {
-#pragma unroll
-        for (int j = 0; j < kProjP; ++j) {
-            float v = acc[i][j];
-            for (int off = 16; off >= 1; off /= 2) v = v + __shfl_down(v, off);
-            if (t % 32 == 0) warp_sums[t / 32][i * kProjP + j] = v;
-        }
-    }
-    __syncthreads();
-    if (t < kProjQ * kProjP) {
-        const int i = t / kProjP;
-        const int j = t % kProjP;
-        const uint32_t q = q0 + i;
-        const uint32_t o = o0 + j;
-        if (q < nq && o < p) {
-            float w[8];
-#pragma unroll
-            for (int g = 0; g < 8; ++g) w[g] = (warp_sums[g][t] + 0.0F) + 0.0F;
-            const float s = ((w[0] + w[4]) + (w[2] + w[6])) + ((w[1] + w[5]) + (w[3] + w[7]));
-            if (!__builtin_isfinite(s)) atomicExch(invalid, 1);
-            y[static_cast<uint64_t>(q) * ld_y + o] = s;
-        }
-    }
+    body::project_fp32_tile(x, ld_x, nq, k, proj, ld_proj, p, y, ld_y, invalid, q0, o0);
 }
 
-template <int N>
-__device__ __forceinline__ void bitonic_sort(float (&v)[N])
+__global__ void __launch_bounds__(64 * kProjWaves)
+project_bf16_kernel(const float* __restrict__ x, uint64_t ld_x, uint32_t nq, uint32_t k,
+                    const __bf16* __restrict__ proj, uint64_t ld_proj, uint32_t p,
+                    float* __restrict__ y, uint64_t ld_y, int* invalid)
 {
-#pragma unroll
-    for (int size = 2; size <= N; size <<= 1) {
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                const int l = i ^ stride;
-                if (l > i) {
-                    const float a = v[i];
-                    const float b = v[l];
-                    const float lo = fminf(a, b);
-                    const float hi = fmaxf(a, b);
-                    if ((i & size) == 0) {
-                        v[i] = lo;
-                        v[l] = hi;
-                    } else {
-                        v[i] = hi;
-                        v[l] = lo;
-                    }
-                }
-            }
-        }
-    }
+    body::project_bf16_tile(x, ld_x, nq, k, proj, ld_proj, p, y, ld_y, invalid, blockIdx.x * 32, blockIdx.y * 32);
 }
-
-constexpr int kMedianThreads = 256;
 
 template <int PMAX>
 __global__ void __launch_bounds__(kMedianThreads)
@@ -137,56 +79,7 @@ median_l1_kernel(const float* __restrict__ refs_t, uint32_t rows, const float* _
                  uint64_t ld_y, uint32_t nq, uint32_t p, unsigned long long* __restrict__ keys,
                  int* invalid)
 {
-    __shared__ float qv[PMAX];
-    __shared__ unsigned long long best_key[kMedianThreads / 64];
-    const uint32_t q = blockIdx.y;
-    if (q >= nq) return;
-    const int t = threadIdx.x;
-    for (uint32_t c = t; c < p; c += kMedianThreads) qv[c] = yq[static_cast<uint64_t>(q) * ld_y + c];
-    __syncthreads();
-    const uint32_t r = blockIdx.x * kMedianThreads + t;
-    unsigned long long key = ~0ULL;
-    if (r < rows) {
-        float v[PMAX];
-        bool bad = false;
-#pragma unroll
-        for (int c = 0; c < PMAX; ++c) {
-            if (static_cast<uint32_t>(c) < p) {
-                float d = __builtin_fabsf(refs_t[static_cast<uint64_t>(c) * rows + r] - qv[c]);
-                if (!__builtin_isfinite(d)) {
-                    bad = true;
-                    d = __builtin_inff();
-                }
-                v[c] = d;
-            } else {
-                v[c] = __builtin_inff();
-            }
-        }
-        bitonic_sort<PMAX>(v);
-        const uint32_t mid = p / 2;
-        float med = 0.0F;
-        float lower = 0.0F;
-#pragma unroll
-        for (int c = 0; c < PMAX; ++c) {
-            if (static_cast<uint32_t>(c) == mid) med = v[c];
-            if (static_cast<uint32_t>(c) + 1U == mid) lower = v[c];
-        }
-        if (p % 2 == 0) med = lower + (med - lower) * 0.5F;
-        if (!__builtin_isfinite(med)) bad = true;
-        if (bad) atomicExch(invalid, 1);
-        key = (static_cast<unsigned long long>(__float_as_uint(med)) << 32) | r;
-    }
-    for (int off = 32; off >= 1; off /= 2) {
-        const unsigned long long other = __shfl_down(key, off);
-        key = other < key ? other : key;
-    }
-    if (t % 64 == 0) best_key[t / 64] = key;
-    __syncthreads();
-    if (t == 0) {
-        unsigned long long b = best_key[0];
-        for (int w = 1; w < kMedianThreads / 64; ++w) b = best_key[w] < b ? best_key[w] : b;
-        atomicMin(keys + q, b);
-    }
+    body::median_block<PMAX>(refs_t, rows, yq, ld_y, nq, p, keys, invalid, blockIdx.y, blockIdx.x);
 }
 
 __global__ void zero_results_kernel(uint64_t* idx, float* dist, uint32_t n)
@@ -213,6 +106,19 @@ void project_fp32(const float* x, uint64_t ld_x, uint32_t nq, uint32_t k, const 
 }
 
 uint32_t median_max_dimension() { return 128; }
+uint64_t project_bf16_rows(uint64_t p) { return (p + 31U) / 32U * 32U; }
+uint64_t project_bf16_ld(uint64_t k) { return (k + 15U) / 16U * 16U + 16U; }
+
+void project_bf16(const float* x, uint64_t ld_x, uint32_t nq, uint32_t k, const void* proj16,
+                  uint64_t ld_proj, uint32_t p, float* y, uint64_t ld_y, int* invalid, hipStream_t stream)
+{
+    if (nq == 0 || p == 0) return;
+    if (ld_x % 4 != 0 || ld_proj % 8 != 0) throw std::invalid_argument("bf16 projection operands are misaligned");
+    const dim3 grid((nq + 31U) / 32U, (p + 31U) / 32U);
+    project_bf16_kernel<<<grid, 64 * kProjWaves, 0, stream>>>(x, ld_x, nq, k, static_cast<const __bf16*>(proj16),
+                                                             ld_proj, p, y, ld_y, invalid);
+    check_launch("bf16 projection kernel");
+}
 
 void median_l1(const float* refs_t, uint32_t rows, const float* yq, uint64_t ld_y, uint32_t nq,
                uint32_t p, unsigned long long* keys, int* invalid, hipStream_t stream)

@@ -14,6 +14,15 @@ void project_fp32(const float* x, uint64_t ld_x, uint32_t nq, uint32_t k, const 
                   uint64_t ld_proj, uint32_t p, float* y, uint64_t ld_y, int* invalid,
                   hipStream_t stream);
 
+// The same projection with x and proj rounded to bf16 on MFMA (binary32
+// accumulation). proj16 is a bf16 matrix of ceil(p/32)*32 rows (rows >= p
+// zero) and ld_proj >= ceil(k/16)*16 elements (columns >= k zero, ld_proj a
+// multiple of 8); ld_x is a multiple of 4.
+void project_bf16(const float* x, uint64_t ld_x, uint32_t nq, uint32_t k, const void* proj16,
+                  uint64_t ld_proj, uint32_t p, float* y, uint64_t ld_y, int* invalid, hipStream_t stream);
+uint64_t project_bf16_rows(uint64_t p);
+uint64_t project_bf16_ld(uint64_t k);
+
 // Median-L1 scan: refs_t is p x rows (transposed projected references), yq
 // nq x p (row stride ld_y). keys[q] receives atomicMin((median bits << 32) |
 // row) and must hold ~0 before the launch. *invalid = 1 on a non-finite

@@ -27,30 +27,13 @@
 #include "common.hpp"
 #include "engine.hpp"
 #include "hier.hpp"
+#include "index.hpp"
 #include "kernels.hpp"
 #include "ultrahigh_ann_amd/build.hpp"
 
 namespace uab = ultrahigh_ann_amd;
 using namespace uhann;
 
-struct uhann_index {
-    int kind = UHANN_KIND_EXACT;
-    int metric = UHANN_METRIC_L2;
-    int device = 0;
-    uint64_t rows = 0;
-    uint64_t dim = 0;
-    uint64_t k = 0;             // sampled coordinates |I|
-    uint64_t k_ld = 0;          // padded row length of gathered queries
-    uint64_t multiplicity = 0;  // sum of multiplicities
-    uint64_t p = 0;             // projection dimension
-    uint64_t p_ld = 0;
-    std::unique_ptr<Scan> scan;  // over dim (exact), k (flat), p (hierarchical L2)
-    DeviceBuffer<uint32_t> cols;
-    DeviceBuffer<float> scales;  // flat L2 importance scales
-    DeviceBuffer<float> proj;    // p x k_ld binary32 projection
-    DeviceBuffer<float> refs_t;  // hierarchical L1: p x rows projected references
-    uint64_t payload_bytes = 0;
-};
 
 namespace {
 
@@ -63,6 +46,15 @@ uab::CoordinateSample to_sample(const uhann_sampled_column* cols, uint64_t count
         s.columns.push_back({cols[i].source_column, cols[i].multiplicity, cols[i].inverse_probability});
     }
     return s;
+}
+
+// binary32 -> bf16 bits, round to nearest even (finite inputs).
+uint16_t bf16_bits(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, sizeof b);
+    b += 0x7fffU + ((b >> 16) & 1U);
+    return static_cast<uint16_t>(b >> 16);
 }
 
 void check_metric(int metric)
@@ -143,8 +135,11 @@ void index_launch(const uhann_index& ix, Workspace& ws, const float* queries, ui
         return;
     }
     // hierarchical
-    if (strategy != UHANN_STRATEGY_DIRECT) {
-        throw std::invalid_argument("hierarchical indexes support the direct strategy only");
+    const bool bf16 = strategy == UHANN_STRATEGY_PROJECTION_BF16;
+    if (strategy != UHANN_STRATEGY_DIRECT && !(bf16 && ix.metric == UHANN_METRIC_L2)) {
+        throw std::invalid_argument(ix.metric == UHANN_METRIC_L2
+                                        ? "hierarchical L2 indexes support the direct and bf16 projection strategies"
+                                        : "hierarchical L1 indexes support the direct strategy only");
     }
     if (ix.rows == 1) {
         kern::zero_results(out_idx, out_dist, n, stream);
@@ -153,8 +148,13 @@ void index_launch(const uhann_index& ix, Workspace& ws, const float* queries, ui
     float* projected = gathered + count * ix.k_ld;
     kern::gather_columns(queries, ld_q, n, ix.cols.data(), nullptr, static_cast<uint32_t>(ix.k), gathered,
                          ix.k_ld, flags, stream);
-    kern::project_fp32(gathered, ix.k_ld, n, static_cast<uint32_t>(ix.k), ix.proj.data(), ix.k_ld,
-                       static_cast<uint32_t>(ix.p), projected, ix.p_ld, flags + 1, stream);
+    if (bf16) {
+        kern::project_bf16(gathered, ix.k_ld, n, static_cast<uint32_t>(ix.k), ix.proj16.data(), ix.proj16_ld,
+                           static_cast<uint32_t>(ix.p), projected, ix.p_ld, flags + 1, stream);
+    } else {
+        kern::project_fp32(gathered, ix.k_ld, n, static_cast<uint32_t>(ix.k), ix.proj.data(), ix.k_ld,
+                           static_cast<uint32_t>(ix.p), projected, ix.p_ld, flags + 1, stream);
+    }
     if (ix.metric == UHANN_METRIC_L2) {
         scan_launch(*ix.scan, projected, ix.p_ld, count, ws.keys.data(), out_idx, out_dist, stream, true,
                     nullptr);
@@ -310,6 +310,17 @@ int uhann_hierarchical_create(int metric, const float* host_refs, uint64_t rows,
             ix->proj = DeviceBuffer<float>(device, 4);
         }
         if (metric == UHANN_METRIC_L2) {
+            // bf16 copy of the binary32 projection (round to nearest even),
+            // zero padded to the MFMA tile.
+            ix->proj16_ld = kern::project_bf16_ld(ix->k);
+            const uint64_t prow = kern::project_bf16_rows(projection_dim);
+            std::vector<uint16_t> p16(checked_mul(prow, ix->proj16_ld, "bf16 projection"), 0);
+            for (uint64_t o = 0; o < projection_dim; ++o) {
+                for (uint64_t j = 0; j < ix->k; ++j) p16[o * ix->proj16_ld + j] = bf16_bits(proj32[o * ix->k + j]);
+            }
+            ix->proj16 = DeviceBuffer<uint16_t>(device, p16.size());
+            check_hip(hipMemcpy(ix->proj16.data(), p16.data(), ix->proj16.bytes(), hipMemcpyHostToDevice),
+                      "cannot copy the bf16 projection to the HIP device");
             ix->scan = std::make_unique<Scan>(metric, projected.data(), rows, projection_dim, nullptr, device);
             // Reference accounting: scan payload + SampledColumn sample + FP32 matrix.
             ix->payload_bytes = ix->scan->payload_bytes + ix->k * sizeof(uhann_sampled_column) +

@@ -10,6 +10,19 @@
 
 namespace uhann::kern {
 
+// Tile shapes of the kernel bodies (bodies.hpp), shared with host code that
+// counts tiles.
+namespace body {
+constexpr int kGatherThreads = 256;
+constexpr int kGatherChunk = 1024;
+constexpr int kGatherRows = 8;  // query rows per gather tile
+constexpr int kProjThreads = 256;
+constexpr int kProjQ = 4;
+constexpr int kProjP = 8;
+constexpr int kProjWaves = 4;  // waves of the bf16 projection tile (32 x 32 outputs)
+constexpr int kMedianThreads = 256;
+}  // namespace body
+
 enum class Metric : int { l2 = 0, l1 = 1 };
 
 // One dense FP32 scan: `nq` query rows against `rows` reference rows, both

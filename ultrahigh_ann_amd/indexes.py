@@ -22,6 +22,7 @@ METRIC_L2 = 0
 METRIC_L1 = 1
 STRATEGY_DIRECT = 0
 STRATEGY_GRAM_BF16 = 1
+STRATEGY_PROJECTION_BF16 = 2
 KIND_EXACT, KIND_FLAT, KIND_HIERARCHICAL = 0, 1, 2
 
 SAMPLE_DTYPE = np.dtype([("source_column", "<u8"), ("multiplicity", "<u8"), ("inverse_probability", "<f8")])
