@@ -33,6 +33,12 @@ sys.path.insert(0, ROOT)
 METRIC = "queries/s at r=7028 d=60660, mixed vs FP32 at equal argmin agreement; p99 ms"
 PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "hbm": 8000.0}
 
+# HBM bytes per launch of the Gram kernel, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy kernel's readings
+# (profiles/exact_l2_r7028_d60660_b4096_pmc_traffic.md). Keyed by
+# (references, dim, batch); other shapes report null.
+GRAM_TRAFFIC_BYTES = {(7028, 60660, 4096): 5.687e9 + 0.155e9}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -216,6 +222,8 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
     steps = 2
 
     def add(name, fn):
+        if args.suite and name not in args.suite.split(","):
+            return
         try:
             res[name] = fn()
             log(f"suite {name}: {res[name]}")
@@ -328,6 +336,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="queries timed on the CPU (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-suite", action="store_true", help="skip the secondary workloads")
+    ap.add_argument("--suite", default="", help="comma-separated secondary workloads to run (default: all)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -343,6 +355,15 @@ def main():
     if not torch.cuda.is_available() or not uh.device_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+
+    if args.copy_calibration:
+        src = torch.ones(256 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB device-to-device copy done")
+        del src, dst
 
     r, d, B = args.rows, args.dim, args.batch
     log(f"generating synthetic data r={r} d={d} B={B} seed={args.seed}")
@@ -400,7 +421,7 @@ def main():
         "peak": PEAK["bf16"],
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK["bf16"], 4),
-        "traffic": None,
+        "traffic": GRAM_TRAFFIC_BYTES.get((r, d, B)),
     }
 
     cpu = None
