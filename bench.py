@@ -212,6 +212,55 @@ def serving(uh, torch, refs_np, queries, args, n_indexes=64):
     return out
 
 
+def mixed_large(uh, torch, args, r=65536, B=8192, sample=16):
+    """Mixed-precision exact L2 at r = 65536 synthetic points (FP32 copy
+    15.9 GB, BF16 copy 8.0 GB), d = 60660, batch 8192: BF16 Gram + certified
+    candidate re-rank vs the FP32 direct scan, argmin agreement and distance
+    error against the binary64 oracle on a query sample (16 threads)."""
+    import oracle
+
+    from ultrahigh_ann_amd import synthetic
+
+    d = args.dim
+    log(f"config r={r} B={B}: generating data")
+    refs, queries = synthetic.log_expression(r, B, d, seed=args.seed + 1, device="cuda")
+    refs_np = refs.cpu().numpy()
+    del refs
+    t0 = time.perf_counter()
+    index = uh.HipExactL2Index(refs_np)
+    index.enable_mixed()
+    ws = index.make_query_workspace(B)
+    log(f"config r={r} B={B}: index + BF16 copy built in {time.perf_counter() - t0:.1f} s")
+    oi = torch.zeros(B, dtype=torch.int64, device="cuda")
+    od = torch.zeros(B, dtype=torch.float32, device="cuda")
+    wall32, ms32 = timed(lambda: index.query_device_async(queries, B, ws, oi, od), 1, 0, torch)
+    idx32 = oi.cpu().numpy().astype(np.uint64)
+    steps = 3
+    wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od, strategy=uh.STRATEGY_GRAM_BF16),
+                     steps, 1, torch)
+    idx16 = oi.cpu().numpy().astype(np.uint64)
+    dist16 = od.cpu().numpy()
+    stats = ws.mixed_stats()
+    t0 = time.perf_counter()
+    e_idx, e_dist = oracle.l2_fp64_scan(refs_np, queries[:sample].cpu().numpy(), cpu_threads())
+    cpu_s = time.perf_counter() - t0
+    rel = np.abs(dist16[:sample].astype(np.float64) - e_dist) / np.maximum(e_dist, 1e-300)
+    out = {"references": r, "dim": d, "batch": B,
+           "fp32_direct": {"queries_per_s": round(B / wall32, 1), "ms_per_batch": round(float(ms32.mean()), 2)},
+           "bf16_mixed": {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
+                          "gram_tflops": round(2.0 * B * r * d / (float(ms.mean()) * 1e-3) / 1e12, 1),
+                          "avg_candidates": round(stats["candidates"] / B, 3),
+                          "escalated": stats["escalated"]},
+           "speedup_mixed_vs_fp32": round((B * steps / wall) / (B / wall32), 2),
+           "same_argmin_mixed_vs_fp32": float(np.mean(idx16 == idx32)),
+           "agreement_fp64": {"sample": sample, "fp32_direct": float(np.mean(idx32[:sample] == e_idx)),
+                              "bf16_mixed": float(np.mean(idx16[:sample] == e_idx)),
+                              "max_rel_distance_error": float(rel.max()), "cpu_seconds": round(cpu_s, 1)}}
+    del index, ws, queries
+    torch.cuda.empty_cache()
+    return out
+
+
 def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
     """Secondary workloads at the same shape; each entry is independent."""
     r, d = refs_np.shape
@@ -315,6 +364,7 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
         return rows
 
     add("serving_64_indexes", lambda: serving(uh, torch, refs_np, queries, args))
+    add("exact_l2_r65536_b8192_mixed", lambda: mixed_large(uh, torch, args))
     add("exact_l1_fp32_direct", exact_l1)
     add("exact_l2_streaming", streaming)
     add("flat_l2", lambda: flat(uh.METRIC_L2))

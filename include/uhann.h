@@ -290,6 +290,17 @@ int uhann_importance_probabilities(int metric, const float* refs, uint64_t rows,
 int uhann_importance_probabilities_device(int metric, const float* host_refs, uint64_t rows,
                                           uint64_t dim, int device, double* out);
 
+/* The mixed strategy's Gram estimates ||q||^2 + ||v||^2 - 2 <bf16(q), bf16(v)>
+ * (binary32 MFMA accumulation) for count device queries against every
+ * reference, written to device_out (count rows of ld_out >= rows floats) on
+ * `stream`. tile selects the Gram block shape (0..7, -1 = the default).
+ * Needs uhann_scan_enable_mixed. No counterpart in the reference (its GEMM
+ * strategy keeps the unrefined SGEMM distances internal): for diagnostics
+ * such as full distance tables and for the tests of each block shape. */
+int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                              uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out,
+                              int tile, void* stream);
+
 /* Autotuner of the direct scan: times every compiled kernel shape (the
  * tiled register tiles and the pair-per-thread scan) with HIP events for
  * batches of 1, 2, 4, ... up to max_batch device queries (row stride

@@ -44,3 +44,10 @@ def uh():
 
 def rng(seed=0):
     return np.random.default_rng(seed)
+
+
+def bf16_round(a):
+    """binary32 -> bf16 (round to nearest even), returned as binary64."""
+    b = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)

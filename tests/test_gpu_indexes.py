@@ -11,7 +11,7 @@ then K2 over p (L2) or the K9 median (L1)."""
 import numpy as np
 import pytest
 
-from conftest import rng
+from conftest import bf16_round, rng
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -221,11 +221,7 @@ def test_hierarchical_parity_bit_exact(uh, oracle_mod, metric_l1, r, d, b, T, p)
     np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
 
 
-def _bf16(a):
-    """binary32 -> bf16 (round to nearest even), returned as binary64."""
-    b = np.ascontiguousarray(a, F32).view(np.uint32).astype(np.uint64)
-    b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
-    return b.astype(np.uint32).view(F32).astype(np.float64)
+_bf16 = bf16_round
 
 
 def test_bf16_rounding_helper():

@@ -12,7 +12,7 @@ one."""
 import numpy as np
 import pytest
 
-from conftest import rng
+from conftest import bf16_round, rng
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -124,3 +124,40 @@ def test_mixed_stage_timing(uh):
     index.query_batch(refs[:16].copy(), 16, ws, uh.STRATEGY_GRAM_BF16)
     t = ws.stage_times()
     assert len(t) == 4 and all(x >= 0 for x in t) and t[1] > 0
+
+
+@pytest.mark.parametrize("r,d,b", [(300, 1000, 70), (517, 3000, 33), (40, 64, 300)])
+def test_gram_estimates_every_tile(uh, r, d, b):
+    """Every Gram block shape gives the same bits (each estimate accumulates
+    the same K steps in the same order), and the estimates are the exact
+    ||q||^2 + ||v||^2 - 2 <bf16(q), bf16(v)> up to binary32 accumulation."""
+    import torch
+
+    g = rng(r * 3 + d)
+    refs = (g.random((r, d), dtype=np.float32) * 4 - 1).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 4 - 1).astype(F32)
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed()
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    outs = []
+    for tile in range(8):
+        out = torch.full((b, r + 3), float("nan"), dtype=torch.float32, device="cuda")
+        index.gram_estimates(dq, b, ws, out, tile)
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    for o in outs:
+        assert np.all(np.isnan(o[:, r:]))  # row stride respected, nothing written past the references
+        np.testing.assert_array_equal(o[:, :r].view(np.uint32), outs[0][:, :r].view(np.uint32))
+    est = outs[0][:, :r].astype(np.float64)
+    qb, vb = bf16_round(q), bf16_round(refs)
+    nq = (q.astype(np.float64) ** 2).sum(1)
+    nv = (refs.astype(np.float64) ** 2).sum(1)
+    want = nq[:, None] + nv[None, :] - 2.0 * (qb @ vb.T)
+    steps = (d + 63) // 64 * 4
+    bound = 2.0 * (steps + 8) * 2.0**-24 * (np.abs(qb) @ np.abs(vb).T) + 4 * 2.0**-24 * (nq[:, None] + nv[None, :])
+    assert np.all(np.abs(est - want) <= bound)
+    # a plausible mistake -- truncating instead of rounding to bf16 -- is outside that bound
+    trunc = lambda a: (np.ascontiguousarray(a, F32).view(np.uint32) & 0xFFFF0000).view(F32).astype(np.float64)
+    wrong = nq[:, None] + nv[None, :] - 2.0 * (trunc(q) @ trunc(refs).T)
+    assert np.any(np.abs(est - wrong) > bound)

@@ -296,6 +296,19 @@ class _DenseScan:
         )
 
 
+    def gram_estimates(self, device_queries, count: int, workspace: QueryWorkspace, out, tile: int = -1,
+                       stream=None, ld_queries=None):
+        """Gram estimates ||q||^2 + ||v||^2 - 2 <bf16(q), bf16(v)> into the
+        device tensor out (count x rows, row stride out.stride(0))."""
+        ld = ld_queries if ld_queries is not None else self.dim
+        check(
+            _native.lib().uhann_scan_gram_estimates(
+                self._h, workspace.handle, c_void_p(device_queries.data_ptr()), ld, count,
+                c_void_p(out.data_ptr()), out.stride(0), tile, _torch_stream_ptr(stream),
+            )
+        )
+
+
 class HipDenseL2ScanIndex(_DenseScan):
     """Exhaustive binary32 squared-L2 scan (reference CudaDenseL2ScanIndex)."""
 

@@ -150,6 +150,8 @@ def _declare(lib):
                                         POINTER(c_int)]),
         "uhann_index_autotune": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, POINTER(c_int), c_int,
                                          POINTER(c_int)]),
+        "uhann_scan_gram_estimates": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_uint64,
+                                              c_int, c_void_p]),
         "uhann_server_create": (c_int, [POINTER(ServerConfig), POINTER(c_void_p)]),
         "uhann_server_destroy": (None, [c_void_p]),
         "uhann_server_register": (c_int, [c_void_p, POINTER(IndexSpec), u64p]),

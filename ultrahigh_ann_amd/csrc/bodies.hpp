@@ -41,6 +41,28 @@ __device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&v)
     }
 }
 
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef float floatx2_t __attribute__((ext_vector_type(2)));
+
+// load_vec with the non-temporal cache policy (streamed data).
+template <int RPT>
+__device__ __forceinline__ void load_vec_stream(const float* __restrict__ p, float (&v)[RPT])
+{
+    if constexpr (RPT == 1) {
+        v[0] = __builtin_nontemporal_load(p);
+    } else if constexpr (RPT == 2) {
+        const floatx2_t x = __builtin_nontemporal_load(reinterpret_cast<const floatx2_t*>(p));
+        v[0] = x[0];
+        v[1] = x[1];
+    } else {
+        const floatx4_t x = __builtin_nontemporal_load(reinterpret_cast<const floatx4_t*>(p));
+        v[0] = x[0];
+        v[1] = x[1];
+        v[2] = x[2];
+        v[3] = x[3];
+    }
+}
+
 template <int METRIC, bool WEIGHTED, bool WFMA>
 __device__ __forceinline__ float accumulate(float acc, float r, float q, float w)
 {
@@ -91,12 +113,17 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
     // tail step a vector may straddle dim (rows are padded to a multiple of 4
     // floats): components at or past dim are skipped, so pad values are
     // never added.
-    auto step = [&](uint32_t col, bool masked) {
+    auto step = [&](uint32_t col, bool masked, bool stream) {
         float rv[TR][RPT];
         float qv[TQ][RPT];
         float wv[RPT];
+        if (stream) {
 #pragma unroll
-        for (int j = 0; j < TR; ++j) load_vec<RPT>(rrow[j] + col, rv[j]);
+            for (int j = 0; j < TR; ++j) load_vec_stream<RPT>(rrow[j] + col, rv[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < TR; ++j) load_vec<RPT>(rrow[j] + col, rv[j]);
+        }
 #pragma unroll
         for (int i = 0; i < TQ; ++i) load_vec<RPT>(qrow[i] + col, qv[i]);
         if constexpr (WEIGHTED) {
@@ -119,9 +146,16 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
 
     const uint32_t col_t = static_cast<uint32_t>(RPT * t);
     const uint32_t full_end = (a.dim / kChains) * kChains;
+    if (a.stream_refs) {
+        // Single-pass streams keep more 256-column steps of loads in flight.
+        constexpr int kStreamUnroll = RPT == 4 && TQ * TR <= 8 ? 4 : 2;
+#pragma unroll kStreamUnroll
+        for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, true);
+    } else {
 #pragma unroll 2
-    for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false);
-    if (full_end + col_t < a.dim) step(full_end + col_t, true);
+        for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, false);
+    }
+    if (full_end + col_t < a.dim) step(full_end + col_t, true, false);
 
     // Replay the reference tree. Cross-thread steps first (chain offsets 16..RPT
     // are thread offsets GROUP/2..1), then the chains inside the thread.

@@ -85,12 +85,17 @@ Scan::Scan(int metric_value, const float* host_refs, uint64_t rows_value, uint64
     if (refs.bytes() != 0) {
         // Row-major with a padded leading dimension (multiple of 4 floats) so
         // the kernels can use 16-byte loads; pad columns are zero and never read.
-        std::vector<float> staged(static_cast<std::size_t>(rows * ld), 0.0F);
-        for (uint64_t r = 0; r < rows; ++r) {
-            std::memcpy(staged.data() + r * ld, host_refs + r * dim, dim * sizeof(float));
+        if (ld == dim) {
+            check_hip(hipMemcpy(refs.data(), host_refs, refs.bytes(), hipMemcpyHostToDevice),
+                      "cannot copy representatives to the HIP device");
+        } else {
+            std::vector<float> staged(static_cast<std::size_t>(rows * ld), 0.0F);
+            for (uint64_t r = 0; r < rows; ++r) {
+                std::memcpy(staged.data() + r * ld, host_refs + r * dim, dim * sizeof(float));
+            }
+            check_hip(hipMemcpy(refs.data(), staged.data(), refs.bytes(), hipMemcpyHostToDevice),
+                      "cannot copy representatives to the HIP device");
         }
-        check_hip(hipMemcpy(refs.data(), staged.data(), refs.bytes(), hipMemcpyHostToDevice),
-                  "cannot copy representatives to the HIP device");
     }
     if (dim != 0 && dim <= kern::scan_pairs_max_dim()) {
         // Short rows also get a transposed copy (dim x rows) for the
@@ -256,6 +261,8 @@ kern::ScanTile choose_tile(uint64_t nq, bool aligned4)
     return t;
 }
 
+bool stream_refs_enabled();
+
 void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t count,
                  unsigned long long* keys, uint64_t* out_idx, float* out_dist,
                  hipStream_t stream, bool queries_padded, const kern::ScanTile* forced_tile)
@@ -299,6 +306,8 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
         throw std::invalid_argument(
             "vector scan tile needs 16-byte aligned, padded query rows");
     }
+    // One query tile reads every reference value exactly once: stream them.
+    a.stream_refs = a.nq <= static_cast<uint32_t>(tile.tq) && stream_refs_enabled();
     if (a.dim == 0) {
         // No coordinates: every distance is zero and the reference returns row 0.
         a.queries = scan.refs.data();
@@ -311,11 +320,19 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
 }
 
 
-// Gram block shape; UHANN_GRAM_TILE (0..3) overrides the default for tuning.
+// Gram block shape; UHANN_GRAM_TILE (0..7) overrides the default for tuning.
+// Non-temporal reference loads for single-pass scans; UHANN_STREAM_REFS=0
+// turns them off (for measurement).
+bool stream_refs_enabled()
+{
+    const char* env = std::getenv("UHANN_STREAM_REFS");
+    return env == nullptr || env[0] != '0';
+}
+
 int gram_tile()
 {
     const char* env = std::getenv("UHANN_GRAM_TILE");
-    if (env != nullptr && env[0] >= '0' && env[0] <= '3' && env[1] == '\0') return env[0] - '0';
+    if (env != nullptr && env[0] >= '0' && env[0] < '0' + kern::gram_tile_count() && env[1] == '\0') return env[0] - '0';
     return 3;
 }
 
@@ -620,6 +637,34 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
         select_device(scan->device);
         scan_launch(*scan, device_queries, ld_queries, count, ws->keys.data(), device_out_idx,
                     device_out_dist, static_cast<hipStream_t>(stream), false, &tile);
+    });
+}
+
+int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                              uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out, int tile,
+                              void* stream)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        checked_ws(scan, ws, count);
+        if (scan->gram == nullptr) throw std::invalid_argument("mixed-precision strategy not enabled for this index");
+        if (tile < -1 || tile > kern::gram_tile_count() - 1) throw std::invalid_argument("unknown Gram tile");
+        if (count == 0) return;
+        if (device_queries == nullptr || device_out == nullptr) {
+            throw std::invalid_argument("device pointers must not be null");
+        }
+        if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
+        if (ld_out < scan->rows) throw std::invalid_argument("output row stride is below the reference count");
+        select_device(scan->device);
+        ws->reserve_mixed(*scan);
+        const GramRefs& g = *scan->gram;
+        const auto s = static_cast<hipStream_t>(stream);
+        const uint32_t n = static_cast<uint32_t>(count);
+        kern::to_bf16_rows(device_queries, ld_queries, n, static_cast<uint32_t>(scan->dim), ws->q16.data(), g.ldk,
+                           ws->qstats.data(), s);
+        kern::gram_bf16(ws->q16.data(), g.ldk, n, g.refs16.data(), g.ldk, static_cast<uint32_t>(scan->rows),
+                        static_cast<uint32_t>(g.ldk), ws->qstats.data(), g.stats.data(), device_out, ld_out,
+                        tile < 0 ? gram_tile() : tile, s);
     });
 }
 

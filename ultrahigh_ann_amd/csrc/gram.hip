@@ -40,6 +40,7 @@ namespace uhann::kern {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBK = 64;
@@ -82,10 +83,8 @@ to_bf16_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t row
     const float* s = src + static_cast<uint64_t>(row) * ld_src;
     __bf16* o = dst + static_cast<uint64_t>(row) * ld_dst;
     double n2 = 0.0, n4 = 0.0, h2 = 0.0, h4 = 0.0, e4 = 0.0;
-    for (uint32_t c = threadIdx.x; c < ld_dst; c += blockDim.x) {
-        const float x = c < dim ? s[c] : 0.0F;
+    auto one = [&](float x) {
         const __bf16 h = static_cast<__bf16>(x);
-        o[c] = h;
         const double xd = x;
         const double hd = static_cast<float>(h);
         const double ed = xd - hd;
@@ -94,7 +93,27 @@ to_bf16_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t row
         h2 += hd * hd;
         h4 += (hd * hd) * (hd * hd);
         e4 += (ed * ed) * (ed * ed);
+        return h;
+    };
+    // Rows of 16-byte aligned source (ld_src % 4 == 0) move four values per
+    // thread and step: one 16-byte load, one 8-byte store. ld_dst is a
+    // multiple of 64, so the destination quads never straddle a row.
+    const bool vec = ld_src % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15U) == 0;
+    uint32_t c0 = 0;
+    if (vec) {
+        const uint32_t full = dim / 4 * 4;
+        for (uint32_t c = threadIdx.x * 4; c < full; c += blockDim.x * 4) {
+            const float4 x = *reinterpret_cast<const float4*>(s + c);
+            bf16x4 h;
+            h[0] = one(x.x);
+            h[1] = one(x.y);
+            h[2] = one(x.z);
+            h[3] = one(x.w);
+            *reinterpret_cast<bf16x4*>(o + c) = h;
+        }
+        c0 = full;
     }
+    for (uint32_t c = c0 + threadIdx.x; c < ld_dst; c += blockDim.x) o[c] = one(c < dim ? s[c] : 0.0F);
     n2 = block_sum_f64(n2, scratch);
     n4 = block_sum_f64(n4, scratch);
     h2 = block_sum_f64(h2, scratch);
@@ -133,29 +152,31 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nblocks)
         *reinterpret_cast<uint4*>(As + (buf) * BM * kLdsRow + a_lds[c]) = ra[c];    \
     _Pragma("unroll") for (int c = 0; c < CB; ++c)                                  \
         *reinterpret_cast<uint4*>(Bs + (buf) * BN * kLdsRow + b_lds[c]) = rb[c];
-// Four K=16 MFMA steps over one LDS stage for the wave's 2 x 2 32x32 tiles.
+// Four K=16 MFMA steps over one LDS stage for the wave's TM x TN 32x32 tiles:
+// TM + TN fragment reads feed TM * TN MFMAs.
 #define UHANN_GRAM_COMPUTE(buf)                                                     \
     _Pragma("unroll") for (int kk = 0; kk < kBK; kk += 16)                          \
     {                                                                               \
         const __bf16* ab = As + (buf) * BM * kLdsRow + a_off + kk;                  \
         const __bf16* bb = Bs + (buf) * BN * kLdsRow + b_off + kk;                  \
-        bf16x8 a0 = *reinterpret_cast<const bf16x8*>(ab);                           \
-        bf16x8 a1 = *reinterpret_cast<const bf16x8*>(ab + 32 * kLdsRow);            \
-        bf16x8 b0 = *reinterpret_cast<const bf16x8*>(bb);                           \
-        bf16x8 b1 = *reinterpret_cast<const bf16x8*>(bb + 32 * kLdsRow);            \
-        acc00 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc00, 0, 0, 0);    \
-        acc01 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc01, 0, 0, 0);    \
-        acc10 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc10, 0, 0, 0);    \
-        acc11 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc11, 0, 0, 0);    \
+        bf16x8 af[TM], bfr[TN];                                                     \
+        _Pragma("unroll") for (int i = 0; i < TM; ++i) af[i] =                      \
+            *reinterpret_cast<const bf16x8*>(ab + i * 32 * kLdsRow);                \
+        _Pragma("unroll") for (int j = 0; j < TN; ++j) bfr[j] =                     \
+            *reinterpret_cast<const bf16x8*>(bb + j * 32 * kLdsRow);                \
+        _Pragma("unroll") for (int i = 0; i < TM; ++i)                              \
+            _Pragma("unroll") for (int j = 0; j < TN; ++j) acc[i][j] =              \
+                __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0); \
     }
 
 // approx[m][n] = nq[m] + nv[n] - 2 <Q[m], V[n]> for m < M, n < N. Q is M x K
 // and V is N x K bf16, row-major with leading dimensions ldq / ldv, K a
 // multiple of kBK (zero padded). A block is a WM x WN grid of waves, each
-// computing a 64 x 64 tile with 2 x 2 v_mfma_f32_32x32x16_bf16. Two LDS
-// stages: while the MFMAs consume one, the next K step travels global ->
-// registers -> the other, so each K step needs a single barrier.
-template <int WM, int WN>
+// computing a (32 TM) x (32 TN) tile with TM x TN v_mfma_f32_32x32x16_bf16
+// accumulators. Two LDS stages: while the MFMAs consume one, the next K step
+// travels global -> registers -> the other, so each K step needs a single
+// barrier.
+template <int WM, int WN, int TM, int TN>
 __global__ void __launch_bounds__(64 * WM * WN)
 gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
                  const __bf16* __restrict__ V, uint64_t ldv, uint32_t N, uint32_t K,
@@ -164,8 +185,9 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
                  uint32_t group_m)
 {
     constexpr int NT = 64 * WM * WN;
-    constexpr int BM = 64 * WM;
-    constexpr int BN = 64 * WN;
+    constexpr int BM = 32 * TM * WM;
+    constexpr int BN = 32 * TN * WN;
+    static_assert((BM * 8) % NT == 0 && (BN * 8) % NT == 0, "tile rows must split evenly over the threads");
     constexpr int CA = BM * 8 / NT;  // 16-byte chunks per thread per K step
     constexpr int CB = BN * 8 / NT;
     __shared__ __attribute__((aligned(16))) __bf16 As[2 * BM * kLdsRow];
@@ -183,8 +205,8 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
     const int tid = threadIdx.x;
     const int lane = tid % 64;
     const int wave = tid / 64;
-    const int wm = (wave / WN) * 64;  // wave's offsets inside the block tile
-    const int wn = (wave % WN) * 64;
+    const int wm = (wave / WN) * 32 * TM;  // wave's offsets inside the block tile
+    const int wn = (wave % WN) * 32 * TN;
 
     uint4 ra[CA], rb[CB];
     const __bf16* qrow[CA];
@@ -207,14 +229,13 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
     const int a_off = (wm + lane % 32) * kLdsRow + (lane / 32) * 8;
     const int b_off = (wn + lane % 32) * kLdsRow + (lane / 32) * 8;
 
-    floatx16 acc00, acc01, acc10, acc11;
+    floatx16 acc[TM][TN];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        acc00[v] = 0.0F;
-        acc01[v] = 0.0F;
-        acc10[v] = 0.0F;
-        acc11[v] = 0.0F;
-    }
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
 
     UHANN_GRAM_LOAD(0)
     UHANN_GRAM_STORE(0)
@@ -231,11 +252,10 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
 
     // D layout of the 32x32 MFMA: lane holds column (lane % 32) and rows
     // 8*(v/4) + 4*(lane/32) + v%4 for its 16 values v.
-    const floatx16* accs[2][2] = {{&acc00, &acc01}, {&acc10, &acc11}};
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < TM; ++i) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < TN; ++j) {
             const uint32_t n = n0 + wn + j * 32 + (lane % 32);
             if (n >= N) continue;
             const float nv = vs[n].norm2;
@@ -244,7 +264,7 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
                 const uint32_t m = m0 + wm + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
                 if (m < M) {
                     const float base = qs[m].norm2 + nv;
-                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * (*accs[i][j])[v];
+                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * acc[i][j][v];
                 }
             }
         }
@@ -280,7 +300,8 @@ __device__ __forceinline__ float unordered(uint32_t u)
 }
 
 constexpr int kSelThreads = 256;
-constexpr int kRerankRows = 8;
+constexpr int kRerankRows = 4;
+constexpr int kKeepLo = 32;  // approx - tau values kept in registers per thread
 
 // One block per query. See the file comment for the algorithm.
 __global__ void __launch_bounds__(kSelThreads)
@@ -306,9 +327,22 @@ select_rerank_kernel(SelectArgs a)
         s_count = 0;
     }
     __syncthreads();
-    // U = min_r (approx + tau)
+    // U = min_r (approx + tau). The lower ends approx - tau of the first
+    // kKeepLo * kSelThreads references stay in registers for the second pass.
     uint32_t local = 0xffffffffU;
-    for (uint32_t r = t; r < a.rows; r += kSelThreads) {
+    float lo_keep[kKeepLo];
+#pragma unroll
+    for (int i = 0; i < kKeepLo; ++i) {
+        const uint32_t r = t + static_cast<uint32_t>(i) * kSelThreads;
+        lo_keep[i] = __builtin_inff();
+        if (r < a.rows) {
+            const float e = arow[r];
+            const float tau = pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
+            local = min(local, ordered(e + tau));
+            lo_keep[i] = e - tau;
+        }
+    }
+    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
         const float up = arow[r] + pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
         local = min(local, ordered(up));
     }
@@ -316,7 +350,14 @@ select_rerank_kernel(SelectArgs a)
     if (t % 64 == 0) atomicMin(&s_min, local);
     __syncthreads();
     const float upper = unordered(s_min);
-    for (uint32_t r = t; r < a.rows; r += kSelThreads) {
+#pragma unroll
+    for (int i = 0; i < kKeepLo; ++i) {
+        if (lo_keep[i] <= upper) {
+            const uint32_t slot = atomicAdd(&s_count, 1U);
+            if (slot < kMaxCandidates) s_cand[slot] = t + static_cast<uint32_t>(i) * kSelThreads;
+        }
+    }
+    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
         const float lo = arow[r] - pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
         if (lo <= upper) {
             const uint32_t slot = atomicAdd(&s_count, 1U);
@@ -351,7 +392,27 @@ select_rerank_kernel(SelectArgs a)
         float acc[kRerankRows];
 #pragma unroll
         for (int j = 0; j < kRerankRows; ++j) acc[j] = 0.0F;
-        for (uint32_t c = t; c < a.dim; c += kSelThreads) {
+        // Chain t adds columns t, t + 256, ... in order; four steps load
+        // together so more loads are in flight per thread.
+        uint32_t c = t;
+        for (; c + 3U * kSelThreads < a.dim; c += 4U * kSelThreads) {
+            float qv[4], rv[4][kRerankRows];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                qv[u] = qrow[c + u * kSelThreads];
+#pragma unroll
+                for (int j = 0; j < kRerankRows; ++j) rv[u][j] = rrow[j][c + u * kSelThreads];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                for (int j = 0; j < kRerankRows; ++j) {
+                    const float d = rv[u][j] - qv[u];
+                    acc[j] = __builtin_fmaf(d, d, acc[j]);
+                }
+            }
+        }
+        for (; c < a.dim; c += kSelThreads) {
             const float qv = qrow[c];
 #pragma unroll
             for (int j = 0; j < kRerankRows; ++j) {
@@ -443,35 +504,42 @@ void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint6
 {
     if (m == 0 || n == 0) return;
     if (k % kBK != 0) throw std::invalid_argument("Gram K must be a multiple of 64");
-    // tile: 0 -> 2x2 waves (128x128), 1 -> 4x2 (256x128), 2 -> 2x4 (128x256),
-    // 3 -> 4x4 (256x256).
-    static const int wm_of[4] = {2, 4, 2, 4};
-    static const int wn_of[4] = {2, 2, 4, 4};
-    if (tile < 0 || tile > 3) throw std::invalid_argument("unknown Gram tile");
-    const uint32_t bm = 64U * wm_of[tile];
-    const uint32_t bn = 64U * wn_of[tile];
+    // tile: waves WM x WN, MFMA tiles per wave TM x TN, block (32 TM WM) x (32 TN WN):
+    //   0: 2x2 waves, 2x2 (128x128)   1: 4x2, 2x2 (256x128)   2: 2x4, 2x2 (128x256)
+    //   3: 4x4, 2x2 (256x256)         4: 2x2, 4x4 (256x256)   5: 2x4, 4x2 (256x256)
+    //   6: 4x2, 2x4 (256x256)         7: 2x2, 4x2 (256x128)
+    struct Shape {
+        int wm, wn, tm, tn;
+    };
+    static const Shape shapes[8] = {{2, 2, 2, 2}, {4, 2, 2, 2}, {2, 4, 2, 2}, {4, 4, 2, 2},
+                                    {2, 2, 4, 4}, {2, 4, 4, 2}, {4, 2, 2, 4}, {2, 2, 4, 2}};
+    if (tile < 0 || tile > 7) throw std::invalid_argument("unknown Gram tile");
+    const Shape sh = shapes[tile];
+    const uint32_t bm = 32U * sh.tm * sh.wm;
+    const uint32_t bn = 32U * sh.tn * sh.wn;
     const uint32_t tiles_m = (m + bm - 1) / bm;
     const uint32_t tiles_n = (n + bn - 1) / bn;
     const uint32_t group_m = 8;
     const dim3 grid(tiles_m * tiles_n);
+    const dim3 block(64 * sh.wm * sh.wn);
+#define UHANN_GRAM_CASE(T, A, B, C, D)                                                                   \
+    case T:                                                                                            \
+        gram_bf16_kernel<A, B, C, D><<<grid, block, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo, \
+                                                                tiles_m, tiles_n, group_m);             \
+        break;
     switch (tile) {
-    case 0:
-        gram_bf16_kernel<2, 2><<<grid, 256, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
-                                                        tiles_m, tiles_n, group_m);
-        break;
-    case 1:
-        gram_bf16_kernel<4, 2><<<grid, 512, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
-                                                        tiles_m, tiles_n, group_m);
-        break;
-    case 2:
-        gram_bf16_kernel<2, 4><<<grid, 512, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
-                                                        tiles_m, tiles_n, group_m);
-        break;
+        UHANN_GRAM_CASE(0, 2, 2, 2, 2)
+        UHANN_GRAM_CASE(1, 4, 2, 2, 2)
+        UHANN_GRAM_CASE(2, 2, 4, 2, 2)
+        UHANN_GRAM_CASE(3, 4, 4, 2, 2)
+        UHANN_GRAM_CASE(4, 2, 2, 4, 4)
+        UHANN_GRAM_CASE(5, 2, 4, 4, 2)
+        UHANN_GRAM_CASE(6, 4, 2, 2, 4)
+        UHANN_GRAM_CASE(7, 2, 2, 4, 2)
     default:
-        gram_bf16_kernel<4, 4><<<grid, 1024, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
-                                                         tiles_m, tiles_n, group_m);
         break;
     }
+#undef UHANN_GRAM_CASE
     check_launch("bf16 Gram kernel");
 }
 
@@ -483,6 +551,7 @@ void select_rerank(const SelectArgs& args, hipStream_t stream)
 }
 
 uint32_t gram_k_padding() { return kBK; }
+int gram_tile_count() { return 8; }
 uint32_t gram_m_tile() { return 256; }
 
 }  // namespace uhann::kern

@@ -57,6 +57,7 @@ void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint6
                int tile, hipStream_t stream);
 void select_rerank(const SelectArgs& args, hipStream_t stream);
 uint32_t gram_k_padding();
+int gram_tile_count();
 uint32_t gram_m_tile();
 
 // Small helpers for the escalation path (scan.hip / gather.hip).

@@ -40,6 +40,10 @@ struct ScanArgs {
     const float* weights = nullptr;
     uint32_t dim = 0;
     unsigned long long* keys = nullptr;
+    // Every reference value is read once by the launch (one query tile):
+    // load references with the non-temporal cache policy so the stream does
+    // not evict the queries from L2.
+    bool stream_refs = false;
 };
 
 // Register tile of the direct scan: a block of 256/rpt threads, each thread

@@ -132,6 +132,10 @@ ScanFn pick(int rpt, int tq, int tr)
     UHANN_TILE(4, 1, 4)
     UHANN_TILE(4, 2, 4)
     UHANN_TILE(4, 4, 4)
+    UHANN_TILE(4, 1, 2)
+    UHANN_TILE(4, 1, 1)
+    UHANN_TILE(4, 2, 2)
+    UHANN_TILE(4, 4, 2)
 #undef UHANN_TILE
     return nullptr;
 }
@@ -152,6 +156,7 @@ const std::vector<ScanTile>& scan_tiles()
         {1, 8, 8, 8},  {1, 12, 12, 8}, {1, 16, 8, 4}, {1, 8, 16, 2}, {1, 4, 16, 1}, {1, 2, 16, 1},
         {1, 1, 16, 1}, {4, 1, 8, 1},   {4, 2, 8, 1},  {4, 4, 8, 1},  {4, 8, 4, 2},  {1, 1, 4, 1},
         {1, 2, 8, 1},  {1, 4, 8, 1},   {1, 16, 4, 4}, {4, 1, 4, 1},  {4, 2, 4, 1},  {4, 4, 4, 1},
+        {4, 1, 2, 1},  {4, 1, 1, 1},   {4, 2, 2, 1},  {4, 4, 2, 1},
     };
     return tiles;
 }
