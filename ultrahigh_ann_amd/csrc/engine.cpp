@@ -332,7 +332,11 @@ bool stream_refs_enabled()
 int gram_tile()
 {
     const char* env = std::getenv("UHANN_GRAM_TILE");
-    if (env != nullptr && env[0] >= '0' && env[0] < '0' + kern::gram_tile_count() && env[1] == '\0') return env[0] - '0';
+    if (env != nullptr) {
+        char* end = nullptr;
+        const long t = std::strtol(env, &end, 10);
+        if (end != env && *end == '\0' && t >= 0 && t < kern::gram_tile_count()) return static_cast<int>(t);
+    }
     return 3;
 }
 
@@ -346,7 +350,12 @@ void Scan::enable_mixed()
     auto g = std::make_unique<GramRefs>();
     const uint64_t step = kern::gram_k_padding();
     g->ldk = std::max<uint64_t>(step, (dim + step - 1) / step * step);
-    g->refs16 = DeviceBuffer<__bf16>(device, checked_mul(rows, g->ldk, "bf16 references"));
+    // Rows padded with zeros to whole Gram tiles: the Gram kernel reads
+    // complete tiles without clamping row indices.
+    const uint64_t mt = kern::gram_m_tile();
+    const uint64_t padded_rows = (rows + mt - 1) / mt * mt;
+    g->refs16 = DeviceBuffer<__bf16>(device, checked_mul(padded_rows, g->ldk, "bf16 references"));
+    check_hip(hipMemset(g->refs16.data(), 0, g->refs16.bytes()), "cannot clear the bf16 references");
     g->stats = DeviceBuffer<kern::RowStats>(device, rows);
     kern::to_bf16_rows(refs.data(), ld, static_cast<uint32_t>(rows), static_cast<uint32_t>(dim),
                        g->refs16.data(), g->ldk, g->stats.data(), nullptr);

@@ -143,15 +143,15 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nblocks)
 // bf16 = ROWS * 8 chunks of 16 bytes; thread tid moves chunks tid + NT*c.
 #define UHANN_GRAM_LOAD(k0)                                                         \
     _Pragma("unroll") for (int c = 0; c < CA; ++c) ra[c] =                          \
-        *reinterpret_cast<const uint4*>(qrow[c] + (k0));                            \
+        *reinterpret_cast<const uint4*>(qbase + c * qstep + (k0));                  \
     _Pragma("unroll") for (int c = 0; c < CB; ++c) rb[c] =                          \
-        *reinterpret_cast<const uint4*>(vrow[c] + (k0));                            \
+        *reinterpret_cast<const uint4*>(vbase + c * vstep + (k0));                  \
     __builtin_amdgcn_sched_barrier(0);
 #define UHANN_GRAM_STORE(buf)                                                       \
     _Pragma("unroll") for (int c = 0; c < CA; ++c)                                  \
-        *reinterpret_cast<uint4*>(As + (buf) * BM * kLdsRow + a_lds[c]) = ra[c];    \
+        *reinterpret_cast<uint4*>(As + (buf) * BM * kLdsRow + lds0 + c * kLdsStep) = ra[c]; \
     _Pragma("unroll") for (int c = 0; c < CB; ++c)                                  \
-        *reinterpret_cast<uint4*>(Bs + (buf) * BN * kLdsRow + b_lds[c]) = rb[c];
+        *reinterpret_cast<uint4*>(Bs + (buf) * BN * kLdsRow + lds0 + c * kLdsStep) = rb[c];
 // Four K=16 MFMA steps over one LDS stage for the wave's TM x TN 32x32 tiles:
 // TM + TN fragment reads feed TM * TN MFMAs.
 #define UHANN_GRAM_COMPUTE(buf)                                                     \
@@ -209,21 +209,15 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
     const int wn = (wave % WN) * 32 * TN;
 
     uint4 ra[CA], rb[CB];
-    const __bf16* qrow[CA];
-    const __bf16* vrow[CB];
-    int a_lds[CA], b_lds[CB];
-#pragma unroll
-    for (int c = 0; c < CA; ++c) {
-        const int idx = tid + NT * c;
-        qrow[c] = Q + static_cast<uint64_t>(min(m0 + idx / 8, M - 1U)) * ldq + (idx % 8) * 8;
-        a_lds[c] = (idx / 8) * kLdsRow + (idx % 8) * 8;
-    }
-#pragma unroll
-    for (int c = 0; c < CB; ++c) {
-        const int idx = tid + NT * c;
-        vrow[c] = V + static_cast<uint64_t>(min(n0 + idx / 8, N - 1U)) * ldv + (idx % 8) * 8;
-        b_lds[c] = (idx / 8) * kLdsRow + (idx % 8) * 8;
-    }
+    // Chunk c of this thread is row tid/8 + c*NT/8, 16-byte column chunk
+    // tid % 8, of the block's tile: one base address per operand plus
+    // compile-time row strides. Q and V hold whole tiles (rows padded).
+    const __bf16* qbase = Q + static_cast<uint64_t>(m0 + tid / 8) * ldq + (tid % 8) * 8;
+    const __bf16* vbase = V + static_cast<uint64_t>(n0 + tid / 8) * ldv + (tid % 8) * 8;
+    const uint64_t qstep = static_cast<uint64_t>(NT / 8) * ldq;
+    const uint64_t vstep = static_cast<uint64_t>(NT / 8) * ldv;
+    const int lds0 = (tid / 8) * kLdsRow + (tid % 8) * 8;
+    constexpr int kLdsStep = (NT / 8) * kLdsRow;
     // MFMA operand fragments: lane holds row (lane % 32), K values
     // 8*(lane/32) .. +7 of each 16-wide K step.
     const int a_off = (wm + lane % 32) * kLdsRow + (lane / 32) * 8;
@@ -513,7 +507,7 @@ void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint6
     };
     static const Shape shapes[8] = {{2, 2, 2, 2}, {4, 2, 2, 2}, {2, 4, 2, 2}, {4, 4, 2, 2},
                                     {2, 2, 4, 4}, {2, 4, 4, 2}, {4, 2, 2, 4}, {2, 2, 4, 2}};
-    if (tile < 0 || tile > 7) throw std::invalid_argument("unknown Gram tile");
+    if (tile < 0 || tile >= gram_tile_count()) throw std::invalid_argument("unknown Gram tile");
     const Shape sh = shapes[tile];
     const uint32_t bm = 32U * sh.tm * sh.wm;
     const uint32_t bn = 32U * sh.tn * sh.wn;

@@ -50,8 +50,11 @@ struct SelectArgs {
 
 void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, __bf16* dst,
                   uint64_t ld_dst, RowStats* stats, hipStream_t stream);
-// tile selects the block shape (0: 128x128, 1: 256x128, 2: 128x256,
-// 3: 256x256 outputs).
+// tile selects the block shape (0 .. gram_tile_count() - 1; block tiles of
+// at most gram_m_tile() x gram_m_tile() outputs). q must hold
+// ceil(m / gram_m_tile()) * gram_m_tile() readable rows and v likewise for n:
+// the kernel reads whole tiles, rows past m / n only feed outputs that are
+// not written.
 void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint64_t ldv, uint32_t n,
                uint32_t k, const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo,
                int tile, hipStream_t stream);
