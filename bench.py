@@ -241,21 +241,35 @@ def mixed_large(uh, torch, args, r=65536, B=8192, sample=16):
     idx16 = oi.cpu().numpy().astype(np.uint64)
     dist16 = od.cpu().numpy()
     stats = ws.mixed_stats()
+    index.enable_mixed(uh.STRATEGY_GRAM_FP8)
+    wall8, ms8 = timed(lambda: index.query_device_async(queries, B, ws, oi, od, strategy=uh.STRATEGY_GRAM_FP8),
+                       steps, 1, torch)
+    idx8 = oi.cpu().numpy().astype(np.uint64)
+    dist8 = od.cpu().numpy()
+    cert8 = ws.shortlist_stats()["certified"]
     t0 = time.perf_counter()
     e_idx, e_dist = oracle.l2_fp64_scan(refs_np, queries[:sample].cpu().numpy(), cpu_threads())
     cpu_s = time.perf_counter() - t0
     rel = np.abs(dist16[:sample].astype(np.float64) - e_dist) / np.maximum(e_dist, 1e-300)
+    rel8 = np.abs(dist8[:sample].astype(np.float64) - e_dist) / np.maximum(e_dist, 1e-300)
     out = {"references": r, "dim": d, "batch": B,
            "fp32_direct": {"queries_per_s": round(B / wall32, 1), "ms_per_batch": round(float(ms32.mean()), 2)},
            "bf16_mixed": {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
                           "gram_tflops": round(2.0 * B * r * d / (float(ms.mean()) * 1e-3) / 1e12, 1),
                           "avg_candidates": round(stats["candidates"] / B, 3),
                           "escalated": stats["escalated"]},
+           "fp8_shortlist": {"queries_per_s": round(B * steps / wall8, 1),
+                             "ms_per_batch": round(float(ms8.mean()), 3),
+                             "shortlist": 16, "same_argmin_as_fp32": float(np.mean(idx8 == idx32)),
+                             "certified_fraction": round(cert8 / B, 4)},
            "speedup_mixed_vs_fp32": round((B * steps / wall) / (B / wall32), 2),
+           "speedup_fp8_vs_fp32": round((B * steps / wall8) / (B / wall32), 2),
            "same_argmin_mixed_vs_fp32": float(np.mean(idx16 == idx32)),
            "agreement_fp64": {"sample": sample, "fp32_direct": float(np.mean(idx32[:sample] == e_idx)),
                               "bf16_mixed": float(np.mean(idx16[:sample] == e_idx)),
-                              "max_rel_distance_error": float(rel.max()), "cpu_seconds": round(cpu_s, 1)}}
+                              "fp8_shortlist": float(np.mean(idx8[:sample] == e_idx)),
+                              "max_rel_distance_error": float(rel.max()),
+                              "max_rel_distance_error_fp8": float(rel8.max()), "cpu_seconds": round(cpu_s, 1)}}
     del index, ws, queries
     torch.cuda.empty_cache()
     return out
@@ -463,6 +477,23 @@ def main():
     ws.set_timing(False)
     stage = np.array(stage).mean(0)
     gemm_ms = float(stage[1])
+
+    # FP8 shortlist strategy on the same batch (not certified; reported beside).
+    index.enable_mixed(uh.STRATEGY_GRAM_FP8)
+    wall8, ms8, idx8 = run(uh.STRATEGY_GRAM_FP8, K, args.warmup, "fp8 shortlist")
+    cert8 = ws.shortlist_stats()["certified"]
+    ws.set_timing(True)
+    stage8 = []
+    for _ in range(3):
+        index.query_device_async(queries, B, ws, out_idx, out_dist, strategy=uh.STRATEGY_GRAM_FP8, stream=stream)
+        stage8.append(ws.stage_times())
+    ws.set_timing(False)
+    stage8 = np.array(stage8).mean(0)
+    fp8_line = {"value": round(B * K / wall8, 2), "unit": "queries/s", "ms_per_step": round(wall8 * 1e3 / K, 3),
+                "same_argmin_as_fp32": float(np.mean(idx8 == fp32_idx)), "certified_fraction": round(cert8 / B, 4),
+                "stages_ms": {"fp8_convert": round(float(stage8[0]), 3), "gram_gemm": round(float(stage8[1]), 3),
+                              "top16_rerank": round(float(stage8[2]), 3)},
+                "gram_tflops": round(2.0 * B * r * d / (float(stage8[1]) * 1e-3) / 1e12, 1)}
     flops = 2.0 * B * r * d
     achieved = flops / (gemm_ms * 1e-3) / 1e12
     roofline = {
@@ -485,6 +516,7 @@ def main():
             "sample": sample,
             "fp32_direct": float(np.mean(fp32_idx[:sample] == fp64_idx)),
             "bf16_mixed": float(np.mean(mixed_idx[:sample] == fp64_idx)),
+            "fp8_shortlist": float(np.mean(idx8[:sample] == fp64_idx)),
         }
         log(f"CPU baseline {cpu['value']:.2f} q/s; agreement with binary64 {agreement}")
 
@@ -523,6 +555,7 @@ def main():
         "mixed_stages_ms": {"bf16_convert": round(float(stage[0]), 3), "gram_gemm": round(gemm_ms, 3),
                             "select_rerank": round(float(stage[2]), 3),
                             "escalation": round(float(stage[3]), 3)},
+        "fp8_shortlist": fp8_line,
         "mixed_counters": {"avg_candidates": round(stats["candidates"] / B, 3),
                            "fp64_reranked": stats["fp64_reranked"], "escalated": stats["escalated"]},
         "agreement_fp64": agreement,

@@ -55,6 +55,14 @@ extern "C" {
  * (cuda_float_projection.cu:234-263); the scan over the projected vectors is
  * the direct binary32 scan. New: the reference projects in binary32 only. */
 #define UHANN_STRATEGY_PROJECTION_BF16 2
+/* L2 only. ||q||^2 + ||v||^2 - 2 s_q s_v q8.v8 with rows quantised to e4m3
+ * under a power-of-two per-row scale, inner products on FP8 MFMA (FP32
+ * accumulation); the 16 smallest estimates per query are re-ranked exactly in
+ * the direct path's binary32 order and binary32 near ties in binary64. The
+ * shortlist is not certified: uhann_workspace_shortlist_stats counts the
+ * queries whose answer the FP8 error bound proves. New (the reference has no
+ * FP8 path). */
+#define UHANN_STRATEGY_GRAM_FP8 3
 
 /* Index kinds reported by uhann_index_info. */
 #define UHANN_KIND_EXACT 0
@@ -139,9 +147,10 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
                                  uint64_t* device_out_idx, float* device_out_dist, int rpt, int tq,
                                  int tr, int group_q, void* stream);
 
-/* Builds the BF16 reference copy and row statistics the GRAM_BF16 strategy
- * needs (rows x dim bf16 + 16 bytes per row of device memory). Call before
- * sharing the handle between threads. L2 only. No counterpart in the
+/* Builds the reference copy and row statistics the GRAM_BF16 strategy (rows x
+ * dim bf16 + 32 bytes per row of device memory) or the GRAM_FP8 strategy
+ * (rows x dim e4m3 bytes + 32 bytes per row) needs. Call before sharing the
+ * handle between threads. L2 only. No counterpart in the
  * reference (its GEMM strategy needs no preparation beyond K1 norms,
  * cuda_dense_l2_scan_index.cu:449-462). */
 int uhann_scan_enable_mixed(uhann_scan* scan, int strategy);
@@ -149,6 +158,12 @@ int uhann_scan_enable_mixed(uhann_scan* scan, int strategy);
 /* Counters of the last GRAM_BF16 batch on this workspace: total candidate
  * set size, queries decided in binary64, queries escalated to the direct
  * scan. Synchronises the workspace stream. */
+/* Of the last GRAM_FP8 batch on this workspace: queries whose answer the FP8
+ * error bound proves equal to the direct path's (no row outside the shortlist
+ * can be closer). Synchronises the workspace stream. No counterpart in the
+ * reference. */
+int uhann_workspace_shortlist_stats(uhann_workspace* ws, uint64_t* certified);
+
 int uhann_workspace_mixed_stats(uhann_workspace* ws, uint64_t* candidates, uint64_t* fp64_reranked,
                                 uint64_t* escalated);
 
@@ -216,7 +231,8 @@ void uhann_index_destroy(uhann_index* index);
 int uhann_index_info(const uhann_index* index, int* kind, int* metric, uint64_t* rows, uint64_t* dim,
                      uint64_t* unique_coordinates, uint64_t* multiplicity, uint64_t* payload_bytes);
 
-/* Prepares GRAM_BF16 for an exact or flat L2 index (see uhann_scan_enable_mixed). */
+/* Prepares GRAM_BF16 or GRAM_FP8 for an exact or flat L2 index (see
+ * uhann_scan_enable_mixed). */
 int uhann_index_enable_mixed(uhann_index* index, int strategy);
 
 /* Replaces make_query_workspace(max_batch) of every facade. */
@@ -300,6 +316,14 @@ int uhann_importance_probabilities_device(int metric, const float* host_refs, ui
 int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
                               uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out,
                               int tile, void* stream);
+
+/* The GRAM_FP8 strategy's estimates ||q||^2 + ||v||^2 - 2 s_q s_v <q8, v8>
+ * for count device queries against every reference, as
+ * uhann_scan_gram_estimates. Needs uhann_scan_enable_mixed(GRAM_FP8). No
+ * counterpart in the reference: for diagnostics and tests. */
+int uhann_scan_fp8_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                             uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out,
+                             void* stream);
 
 /* Autotuner of the direct scan: times every compiled kernel shape (the
  * tiled register tiles and the pair-per-thread scan) with HIP events for

@@ -58,6 +58,7 @@ METRIC_L1 = 1
 STRATEGY_DIRECT = 0
 STRATEGY_GRAM_BF16 = 1
 STRATEGY_PROJECTION_BF16 = 2
+STRATEGY_GRAM_FP8 = 3
 
 __all__ = [
     "METRIC_L1",
@@ -65,6 +66,7 @@ __all__ = [
     "STRATEGY_DIRECT",
     "STRATEGY_GRAM_BF16",
     "STRATEGY_PROJECTION_BF16",
+    "STRATEGY_GRAM_FP8",
     "HipDenseL1ScanIndex",
     "HipExactIndex",
     "HipFlatL1AnnIndex",
@@ -149,6 +151,12 @@ class QueryWorkspace:
         ms = np.zeros(4, np.float32)
         check(_native.lib().uhann_workspace_stage_times(self._h, f32ptr(ms), 4))
         return [float(x) for x in ms]
+
+    def shortlist_stats(self) -> dict:
+        """Certified queries of the last GRAM_FP8 batch."""
+        c = c_uint64()
+        check(_native.lib().uhann_workspace_shortlist_stats(self._h, ctypes.byref(c)))
+        return {"certified": c.value}
 
     def mixed_stats(self) -> dict:
         c, f, e = c_uint64(), c_uint64(), c_uint64()
@@ -305,6 +313,19 @@ class _DenseScan:
             _native.lib().uhann_scan_gram_estimates(
                 self._h, workspace.handle, c_void_p(device_queries.data_ptr()), ld, count,
                 c_void_p(out.data_ptr()), out.stride(0), tile, _torch_stream_ptr(stream),
+            )
+        )
+
+
+    def fp8_estimates(self, device_queries, count: int, workspace: QueryWorkspace, out, stream=None,
+                      ld_queries=None):
+        """FP8 estimates ||q||^2 + ||v||^2 - 2 s_q s_v <q8, v8> into the device
+        tensor out (count x rows, row stride out.stride(0))."""
+        ld = ld_queries if ld_queries is not None else self.dim
+        check(
+            _native.lib().uhann_scan_fp8_estimates(
+                self._h, workspace.handle, c_void_p(device_queries.data_ptr()), ld, count,
+                c_void_p(out.data_ptr()), out.stride(0), _torch_stream_ptr(stream),
             )
         )
 

@@ -340,20 +340,38 @@ int gram_tile()
     return 3;
 }
 
-void Scan::enable_mixed()
+void Scan::enable_mixed(int strategy)
 {
+    if (strategy != UHANN_STRATEGY_GRAM_BF16 && strategy != UHANN_STRATEGY_GRAM_FP8) {
+        throw std::invalid_argument("unknown mixed strategy");
+    }
     if (metric != UHANN_METRIC_L2) {
         throw std::invalid_argument("the mixed-precision strategy applies to the L2 metric only");
     }
-    if (gram) return;
     select_device(device);
-    auto g = std::make_unique<GramRefs>();
-    const uint64_t step = kern::gram_k_padding();
-    g->ldk = std::max<uint64_t>(step, (dim + step - 1) / step * step);
-    // Rows padded with zeros to whole Gram tiles: the Gram kernel reads
+    // Rows padded with zeros to whole Gram tiles: the Gram kernels read
     // complete tiles without clamping row indices.
     const uint64_t mt = kern::gram_m_tile();
     const uint64_t padded_rows = (rows + mt - 1) / mt * mt;
+    if (strategy == UHANN_STRATEGY_GRAM_FP8) {
+        if (fp8) return;
+        auto f = std::make_unique<Fp8Refs>();
+        const uint64_t step = kern::gram_fp8_k_padding();
+        f->ldk8 = std::max<uint64_t>(step, (dim + step - 1) / step * step);
+        f->refs8 = DeviceBuffer<uint8_t>(device, checked_mul(padded_rows, f->ldk8, "fp8 references"));
+        check_hip(hipMemset(f->refs8.data(), 0, f->refs8.bytes()), "cannot clear the fp8 references");
+        f->stats = DeviceBuffer<kern::RowStats>(device, rows);
+        kern::to_fp8_rows(refs.data(), ld, static_cast<uint32_t>(rows), static_cast<uint32_t>(dim), f->refs8.data(),
+                          f->ldk8, f->stats.data(), nullptr);
+        check_hip(hipDeviceSynchronize(), "fp8 reference conversion failed");
+        payload_bytes += f->refs8.bytes() + f->stats.bytes();
+        fp8 = std::move(f);
+        return;
+    }
+    if (gram) return;
+    auto g = std::make_unique<GramRefs>();
+    const uint64_t step = kern::gram_k_padding();
+    g->ldk = std::max<uint64_t>(step, (dim + step - 1) / step * step);
     g->refs16 = DeviceBuffer<__bf16>(device, checked_mul(padded_rows, g->ldk, "bf16 references"));
     check_hip(hipMemset(g->refs16.data(), 0, g->refs16.bytes()), "cannot clear the bf16 references");
     g->stats = DeviceBuffer<kern::RowStats>(device, rows);
@@ -366,19 +384,38 @@ void Scan::enable_mixed()
 
 void Workspace::reserve_mixed(const Scan& scan)
 {
-    if (approx.bytes() != 0 || scan.gram == nullptr) return;
+    if (scan.gram == nullptr || q16.bytes() != 0) return;
     const uint64_t mt = kern::gram_m_tile();
     const uint64_t mrows = (max_batch + mt - 1) / mt * mt;
     q16 = DeviceBuffer<__bf16>(device, checked_mul(mrows, scan.gram->ldk, "bf16 query workspace"));
     qstats = DeviceBuffer<kern::RowStats>(device, mrows);
+    payload_bytes += q16.bytes() + qstats.bytes();
+    reserve_common(scan);
+}
+
+void Workspace::reserve_fp8(const Scan& scan)
+{
+    if (scan.fp8 == nullptr || q8.bytes() != 0) return;
+    const uint64_t mt = kern::gram_m_tile();
+    const uint64_t mrows = (max_batch + mt - 1) / mt * mt;
+    q8 = DeviceBuffer<uint8_t>(device, checked_mul(mrows, scan.fp8->ldk8, "fp8 query workspace"));
+    q8stats = DeviceBuffer<kern::RowStats>(device, mrows);
+    payload_bytes += q8.bytes() + q8stats.bytes();
+    reserve_common(scan);
+}
+
+// Estimate matrix, counters and escalation buffers shared by both strategies.
+void Workspace::reserve_common(const Scan& scan)
+{
+    if (approx.bytes() != 0) return;
     approx = DeviceBuffer<float>(device, checked_mul(max_batch, scan.rows, "Gram workspace"));
     counters = DeviceBuffer<unsigned int>(device, 8);
     escalate_list = DeviceBuffer<unsigned int>(device, max_batch);
     esc_queries = DeviceBuffer<float>(device, checked_mul(max_batch, scan.ld, "escalation workspace"));
     esc_idx = DeviceBuffer<uint64_t>(device, max_batch);
     esc_dist = DeviceBuffer<float>(device, max_batch);
-    payload_bytes += q16.bytes() + qstats.bytes() + approx.bytes() + counters.bytes() +
-                     escalate_list.bytes() + esc_queries.bytes() + esc_idx.bytes() + esc_dist.bytes();
+    payload_bytes += approx.bytes() + counters.bytes() + escalate_list.bytes() + esc_queries.bytes() +
+                     esc_idx.bytes() + esc_dist.bytes();
 }
 
 void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q,
@@ -410,7 +447,9 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
     a.ld_approx = scan.rows;
     a.qstats = ws.qstats.data();
     a.vstats = g.stats.data();
-    a.acc_steps_sqrt = static_cast<float>(std::sqrt(static_cast<double>((g.ldk + 15) / 16)));
+    // BF16 MFMA: binary32 rounding once per 16-product step, modelled as
+    // random (Hoeffding, K = 7) over ceil(K/16) steps.
+    a.acc_coef = 7.0F * static_cast<float>(std::sqrt(static_cast<double>((g.ldk + 15) / 16)));
     a.queries = queries;
     a.ld_q = ld_q;
     a.refs = scan.refs.data();
@@ -449,6 +488,60 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
     }
 }
 
+void fp8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, uint64_t count,
+                uint64_t* out_idx, float* out_dist, hipStream_t stream)
+{
+    if (scan.fp8 == nullptr) throw std::invalid_argument("fp8 strategy not enabled for this index");
+    if (out_idx == nullptr || out_dist == nullptr) {
+        throw std::invalid_argument("the fp8 strategy needs index and distance outputs");
+    }
+    ws.reserve_fp8(scan);
+    const Fp8Refs& f = *scan.fp8;
+    const uint32_t n = static_cast<uint32_t>(count);
+    ws.mark(0, stream);
+    kern::to_fp8_rows(queries, ld_q, n, static_cast<uint32_t>(scan.dim), ws.q8.data(), f.ldk8, ws.q8stats.data(),
+                      stream);
+    ws.mark(1, stream);
+    kern::gram_fp8(ws.q8.data(), f.ldk8, n, f.refs8.data(), f.ldk8, static_cast<uint32_t>(scan.rows),
+                   static_cast<uint32_t>(f.ldk8), ws.q8stats.data(), f.stats.data(), ws.approx.data(), scan.rows,
+                   stream);
+    ws.mark(2, stream);
+    kern::zero_u32(ws.counters.data(), 8, stream);
+    kern::SelectArgs a;
+    a.nq = n;
+    a.rows = static_cast<uint32_t>(scan.rows);
+    a.dim = static_cast<uint32_t>(scan.dim);
+    a.approx = ws.approx.data();
+    a.ld_approx = scan.rows;
+    a.qstats = ws.q8stats.data();
+    a.vstats = f.stats.data();
+    // FP8 MFMA: its sum of 64 products per instruction is coarser than
+    // binary32 (errors up to 2.4 u_32 per product relative to sum |q8 v8|
+    // were measured), so the bound allows 8 u_32 per product.
+    a.acc_coef = 8.0F * static_cast<float>(f.ldk8);
+    a.queries = queries;
+    a.ld_q = ld_q;
+    a.refs = scan.refs.data();
+    a.ld_ref = scan.ld;
+    const double k = std::ceil(static_cast<double>(scan.dim) / 256.0) + 10.0;
+    const double u = std::ldexp(1.0, -24);
+    const double gamma = k * u / (1.0 - k * u);
+    a.fp32_band = (1.0 + gamma) / (1.0 - gamma);
+    a.out_idx = out_idx;
+    a.out_dist = out_dist;
+    a.stats = reinterpret_cast<kern::MixedStats*>(ws.counters.data() + 4);
+    kern::select_topk_rerank(a, stream);
+    ws.mark(3, stream);
+    ws.mark(4, stream);
+    if (ws.timing) {
+        check_hip(hipEventSynchronize(ws.events[4]), "HIP event wait failed");
+        for (int i = 0; i < Workspace::kStages; ++i) {
+            check_hip(hipEventElapsedTime(&ws.stage_ms[i], ws.events[i], ws.events[i + 1]),
+                      "cannot read HIP event time");
+        }
+    }
+}
+
 }  // namespace uhann
 
 using namespace uhann;
@@ -476,6 +569,8 @@ void run_strategy(const Scan& scan, Workspace& ws, const float* queries, uint64_
                     nullptr);
     } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
         mixed_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
+    } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {
+        fp8_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
     } else {
         throw std::invalid_argument("unknown query strategy");
     }
@@ -677,6 +772,31 @@ int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const
     });
 }
 
+int uhann_scan_fp8_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                             uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out, void* stream)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        checked_ws(scan, ws, count);
+        if (scan->fp8 == nullptr) throw std::invalid_argument("fp8 strategy not enabled for this index");
+        if (count == 0) return;
+        if (device_queries == nullptr || device_out == nullptr) {
+            throw std::invalid_argument("device pointers must not be null");
+        }
+        if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
+        if (ld_out < scan->rows) throw std::invalid_argument("output row stride is below the reference count");
+        select_device(scan->device);
+        ws->reserve_fp8(*scan);
+        const Fp8Refs& f = *scan->fp8;
+        const auto s = static_cast<hipStream_t>(stream);
+        const uint32_t n = static_cast<uint32_t>(count);
+        kern::to_fp8_rows(device_queries, ld_queries, n, static_cast<uint32_t>(scan->dim), ws->q8.data(), f.ldk8,
+                          ws->q8stats.data(), s);
+        kern::gram_fp8(ws->q8.data(), f.ldk8, n, f.refs8.data(), f.ldk8, static_cast<uint32_t>(scan->rows),
+                       static_cast<uint32_t>(f.ldk8), ws->q8stats.data(), f.stats.data(), device_out, ld_out, s);
+    });
+}
+
 int uhann_scan_autotune(uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
                         uint64_t ld_queries, uint64_t max_batch, int* table, int capacity, int* count)
 {
@@ -711,8 +831,7 @@ int uhann_scan_enable_mixed(uhann_scan* scan, int strategy)
 {
     return guarded([&] {
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
-        if (strategy != UHANN_STRATEGY_GRAM_BF16) throw std::invalid_argument("unknown mixed strategy");
-        scan->enable_mixed();
+        scan->enable_mixed(strategy);
     });
 }
 
@@ -731,6 +850,21 @@ int uhann_workspace_mixed_stats(uhann_workspace* ws, uint64_t* candidates, uint6
         if (candidates) *candidates = host[4];
         if (fp64_reranked) *fp64_reranked = host[5];
         if (escalated) *escalated = host[6];
+    });
+}
+
+int uhann_workspace_shortlist_stats(uhann_workspace* ws, uint64_t* certified)
+{
+    return guarded([&] {
+        if (ws == nullptr) throw std::invalid_argument("workspace handle must not be null");
+        unsigned int host[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (ws->counters.bytes() != 0) {
+            select_device(ws->device);
+            check_hip(hipStreamSynchronize(ws->stream), "workspace stream failed");
+            check_hip(hipMemcpy(host, ws->counters.data(), sizeof(host), hipMemcpyDeviceToHost),
+                      "cannot read shortlist statistics");
+        }
+        if (certified) *certified = host[7];
     });
 }
 

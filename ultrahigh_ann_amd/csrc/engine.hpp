@@ -27,6 +27,15 @@ struct GramRefs {
     DeviceBuffer<kern::RowStats> stats;
 };
 
+// Per-row-scaled e4m3 copy of the references for GRAM_FP8: rows padded to
+// whole Gram tiles, ldk8 = dim rounded up to the FP8 K step (bytes), plus
+// row statistics (scale included).
+struct Fp8Refs {
+    uint64_t ldk8 = 0;
+    DeviceBuffer<uint8_t> refs8;
+    DeviceBuffer<kern::RowStats> stats;
+};
+
 // Dense FP32 references resident on one device, row-major with a leading
 // dimension padded to a multiple of 4 floats.
 struct Scan {
@@ -46,7 +55,8 @@ struct Scan {
     DeviceBuffer<float> weights;
     uint64_t payload_bytes = 0;
     std::string device_name;
-    std::unique_ptr<GramRefs> gram;  // set by enable_mixed()
+    std::unique_ptr<GramRefs> gram;  // set by enable_mixed(GRAM_BF16)
+    std::unique_ptr<Fp8Refs> fp8;    // set by enable_mixed(GRAM_FP8)
 
     // Autotuned direct-scan kernels per batch bucket b (batches of
     // 2^(b-1)+1 .. 2^b queries); empty entries fall back to the heuristic.
@@ -57,7 +67,7 @@ struct Scan {
     };
     Tuned tuned[24];
 
-    void enable_mixed();
+    void enable_mixed(int strategy);
 };
 
 // Per-caller device buffers and a private stream. `owner` identifies the
@@ -91,7 +101,11 @@ struct Workspace {
     DeviceBuffer<float> esc_queries;
     DeviceBuffer<uint64_t> esc_idx;
     DeviceBuffer<float> esc_dist;
+    DeviceBuffer<uint8_t> q8;           // FP8 strategy: e4m3 queries
+    DeviceBuffer<kern::RowStats> q8stats;
     void reserve_mixed(const Scan& scan);
+    void reserve_fp8(const Scan& scan);
+    void reserve_common(const Scan& scan);
 
     // Optional stage timing (HIP events on the launch stream):
     // [0] bf16 conversion, [1] Gram GEMM, [2] select + re-rank, [3] escalation.
@@ -124,6 +138,11 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
 // `stream` once to read the escalation count.
 void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q,
                   uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream);
+
+// FP8 shortlist query (e4m3 Gram -> top-16 per query -> exact re-rank ->
+// binary64 near ties); no host synchronisation.
+void fp8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, uint64_t count,
+                uint64_t* out_idx, float* out_dist, hipStream_t stream);
 
 }  // namespace uhann
 

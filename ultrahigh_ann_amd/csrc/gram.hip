@@ -127,6 +127,7 @@ to_bf16_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t row
         st.l4_hat = static_cast<float>(sqrt(sqrt(h4))) * 1.0001F;
         st.l2_hat = static_cast<float>(sqrt(h2)) * 1.0001F;
         st.d4 = static_cast<float>(sqrt(sqrt(e4))) * 1.0001F;
+        st.scale = 1.0F;
         stats[row] = st;
     }
 }
@@ -274,11 +275,12 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
 // c_i = |q_i dv_i| + |v^_i dq_i| and Minkowski + Cauchy-Schwarz
 // (sum c_i^2 <= (||q||_4 ||dv||_4 + ||v^||_4 ||dq||_4)^2) bound the error by
 // K (||q||_4 ||dv||_4 + ||v^||_4 ||dq||_4) except with probability
-// 2 exp(-K^2/2); dv and dq are the rows' actual rounding errors.
-__device__ __forceinline__ float pair_tau(const RowStats& q, const RowStats& v, float acc_steps_sqrt)
+// 2 exp(-K^2/2); dv and dq are the rows' actual rounding errors. The MFMA
+// accumulation adds acc_coef u_32 ||q^||_2 ||v^||_2 (acc_coef set by the
+// caller for the instruction's accumulation behaviour).
+__device__ __forceinline__ float pair_tau(const RowStats& q, const RowStats& v, float acc_coef)
 {
-    const float dot_bound = kHoeffdingK * (q.l4 * v.d4 + v.l4_hat * q.d4 +
-                                           kU32 * acc_steps_sqrt * q.l2_hat * v.l2_hat);
+    const float dot_bound = kHoeffdingK * (q.l4 * v.d4 + v.l4_hat * q.d4) + acc_coef * kU32 * q.l2_hat * v.l2_hat;
     return 2.0F * dot_bound + 4.0F * kU32 * (q.norm2 + v.norm2);
 }
 
@@ -298,81 +300,15 @@ constexpr int kRerankRows = 4;
 constexpr int kKeepLo = 32;  // approx - tau values kept in registers per thread
 
 // One block per query. See the file comment for the algorithm.
-__global__ void __launch_bounds__(kSelThreads)
-select_rerank_kernel(SelectArgs a)
+// Exact re-rank of `count` candidate rows s_cand of query q: binary32
+// distances in the reference's order (thread t is chain t) into s_dist, the
+// lowest (distance, row), then the binary64 minimum among candidates inside
+// the binary32 error band. Uses the whole block.
+__device__ void rerank_pick(const SelectArgs& a, uint32_t q, const uint32_t* s_cand, uint32_t count, float* s_dist,
+                            float (*s_group)[kRerankRows], double* s_scratch, uint32_t* s_best, float* s_best_d,
+                            uint32_t& best, float& best_d)
 {
-    __shared__ uint32_t s_min;
-    __shared__ uint32_t s_count;
-    __shared__ uint32_t s_cand[kMaxCandidates];
-    __shared__ float s_dist[kMaxCandidates];
-    __shared__ float s_group[8][kRerankRows];
-    __shared__ double s_scratch[8];
-    __shared__ uint32_t s_best;
-    __shared__ float s_best_d;
-
-    const uint32_t q = blockIdx.x;
-    if (q >= a.nq) return;
     const int t = threadIdx.x;
-    const RowStats qs = a.qstats[q];
-    const float* arow = a.approx + static_cast<uint64_t>(q) * a.ld_approx;
-
-    if (t == 0) {
-        s_min = 0xffffffffU;
-        s_count = 0;
-    }
-    __syncthreads();
-    // U = min_r (approx + tau). The lower ends approx - tau of the first
-    // kKeepLo * kSelThreads references stay in registers for the second pass.
-    uint32_t local = 0xffffffffU;
-    float lo_keep[kKeepLo];
-#pragma unroll
-    for (int i = 0; i < kKeepLo; ++i) {
-        const uint32_t r = t + static_cast<uint32_t>(i) * kSelThreads;
-        lo_keep[i] = __builtin_inff();
-        if (r < a.rows) {
-            const float e = arow[r];
-            const float tau = pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
-            local = min(local, ordered(e + tau));
-            lo_keep[i] = e - tau;
-        }
-    }
-    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
-        const float up = arow[r] + pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
-        local = min(local, ordered(up));
-    }
-    for (int off = 32; off >= 1; off /= 2) local = min(local, __shfl_down(local, off));
-    if (t % 64 == 0) atomicMin(&s_min, local);
-    __syncthreads();
-    const float upper = unordered(s_min);
-#pragma unroll
-    for (int i = 0; i < kKeepLo; ++i) {
-        if (lo_keep[i] <= upper) {
-            const uint32_t slot = atomicAdd(&s_count, 1U);
-            if (slot < kMaxCandidates) s_cand[slot] = t + static_cast<uint32_t>(i) * kSelThreads;
-        }
-    }
-    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
-        const float lo = arow[r] - pair_tau(qs, a.vstats[r], a.acc_steps_sqrt);
-        if (lo <= upper) {
-            const uint32_t slot = atomicAdd(&s_count, 1U);
-            if (slot < kMaxCandidates) s_cand[slot] = r;
-        }
-    }
-    __syncthreads();
-    const uint32_t count = s_count;
-    // No candidate only happens with non-finite estimates (overflowing
-    // norms); both cases fall back to the exact scan.
-    if (count > kMaxCandidates || count == 0) {
-        if (t == 0) {
-            const uint32_t slot = atomicAdd(a.escalate_count, 1U);
-            a.escalate_list[slot] = q;
-            a.out_idx[q] = 0;
-            a.out_dist[q] = 0.0F;
-            if (a.stats != nullptr) atomicAdd(&a.stats->escalated, 1U);
-        }
-        return;
-    }
-
     // Exact binary32 re-rank in the reference's order: thread t is chain t.
     const float* qrow = a.queries + static_cast<uint64_t>(q) * a.ld_q;
     for (uint32_t g = 0; g < count; g += kRerankRows) {
@@ -430,22 +366,22 @@ select_rerank_kernel(SelectArgs a)
     }
 
     if (t == 0) {
-        uint32_t best = 0xffffffffU;
-        float best_d = __uint_as_float(0x7f800000U);
+        uint32_t b = 0xffffffffU;
+        float b_d = __uint_as_float(0x7f800000U);
         for (uint32_t i = 0; i < count; ++i) {
             const float d = s_dist[i];
             const uint32_t r = s_cand[i];
-            if (d < best_d || (d == best_d && r < best)) {
-                best_d = d;
-                best = r;
+            if (d < b_d || (d == b_d && r < b)) {
+                b_d = d;
+                b = r;
             }
         }
-        s_best = best;
-        s_best_d = best_d;
+        *s_best = b;
+        *s_best_d = b_d;
     }
     __syncthreads();
-    uint32_t best = s_best;
-    float best_d = s_best_d;
+    best = *s_best;
+    best_d = *s_best_d;
 
     // Binary64 re-rank of candidates inside the binary32 error band.
     const double band = static_cast<double>(best_d) * a.fp32_band;
@@ -475,10 +411,430 @@ select_rerank_kernel(SelectArgs a)
         best_d = best64_f;
         if (t == 0 && a.stats != nullptr) atomicAdd(&a.stats->fp64_reranked, 1U);
     }
+}
+
+__global__ void __launch_bounds__(kSelThreads)
+select_rerank_kernel(SelectArgs a)
+{
+    __shared__ uint32_t s_min;
+    __shared__ uint32_t s_count;
+    __shared__ uint32_t s_cand[kMaxCandidates];
+    __shared__ float s_dist[kMaxCandidates];
+    __shared__ float s_group[8][kRerankRows];
+    __shared__ double s_scratch[8];
+    __shared__ uint32_t s_best;
+    __shared__ float s_best_d;
+
+    const uint32_t q = blockIdx.x;
+    if (q >= a.nq) return;
+    const int t = threadIdx.x;
+    const RowStats qs = a.qstats[q];
+    const float* arow = a.approx + static_cast<uint64_t>(q) * a.ld_approx;
+
+    if (t == 0) {
+        s_min = 0xffffffffU;
+        s_count = 0;
+    }
+    __syncthreads();
+    // U = min_r (approx + tau). The lower ends approx - tau of the first
+    // kKeepLo * kSelThreads references stay in registers for the second pass.
+    uint32_t local = 0xffffffffU;
+    float lo_keep[kKeepLo];
+#pragma unroll
+    for (int i = 0; i < kKeepLo; ++i) {
+        const uint32_t r = t + static_cast<uint32_t>(i) * kSelThreads;
+        lo_keep[i] = __builtin_inff();
+        if (r < a.rows) {
+            const float e = arow[r];
+            const float tau = pair_tau(qs, a.vstats[r], a.acc_coef);
+            local = min(local, ordered(e + tau));
+            lo_keep[i] = e - tau;
+        }
+    }
+    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
+        const float up = arow[r] + pair_tau(qs, a.vstats[r], a.acc_coef);
+        local = min(local, ordered(up));
+    }
+    for (int off = 32; off >= 1; off /= 2) local = min(local, __shfl_down(local, off));
+    if (t % 64 == 0) atomicMin(&s_min, local);
+    __syncthreads();
+    const float upper = unordered(s_min);
+#pragma unroll
+    for (int i = 0; i < kKeepLo; ++i) {
+        if (lo_keep[i] <= upper) {
+            const uint32_t slot = atomicAdd(&s_count, 1U);
+            if (slot < kMaxCandidates) s_cand[slot] = t + static_cast<uint32_t>(i) * kSelThreads;
+        }
+    }
+    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
+        const float lo = arow[r] - pair_tau(qs, a.vstats[r], a.acc_coef);
+        if (lo <= upper) {
+            const uint32_t slot = atomicAdd(&s_count, 1U);
+            if (slot < kMaxCandidates) s_cand[slot] = r;
+        }
+    }
+    __syncthreads();
+    const uint32_t count = s_count;
+    // No candidate only happens with non-finite estimates (overflowing
+    // norms); both cases fall back to the exact scan.
+    if (count > kMaxCandidates || count == 0) {
+        if (t == 0) {
+            const uint32_t slot = atomicAdd(a.escalate_count, 1U);
+            a.escalate_list[slot] = q;
+            a.out_idx[q] = 0;
+            a.out_dist[q] = 0.0F;
+            if (a.stats != nullptr) atomicAdd(&a.stats->escalated, 1U);
+        }
+        return;
+    }
+
+    uint32_t best;
+    float best_d;
+    rerank_pick(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
     if (t == 0) {
         a.out_idx[q] = best;
         a.out_dist[q] = best_d;
         if (a.stats != nullptr) atomicAdd(&a.stats->candidates, count);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Per-row-scaled FP8 (e4m3) Gram + top-k shortlist re-rank.
+
+constexpr int kBK8 = 128;             // e4m3 values (bytes) per K step
+constexpr int kLdsRow8 = kBK8 + 16;   // LDS row stride in bytes (padding)
+
+// e4m3 (OCP: bias 7, max 448, no infinities) of |v| <= 448, round to
+// nearest even. Normal results keep 3 mantissa bits; below 2^-6 the result
+// is a multiple of 2^-9 (m = 8 rounds up to the smallest normal, 0x08).
+__device__ __forceinline__ uint32_t e4m3_rne(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    const uint32_t sign = (u >> 24) & 0x80U;
+    u &= 0x7fffffffU;
+    if (u >= 0x7f800000U) return sign | 0x7fU;  // NaN stays NaN
+    if (u >= (121U << 23)) {                       // |v| >= 2^-6
+        const uint32_t r = u + 0x7ffffU + ((u >> 20) & 1U);
+        return sign | ((((r >> 23) - 120U) & 0xfU) << 3) | ((r >> 20) & 7U);
+    }
+    const uint32_t m = static_cast<uint32_t>(rintf(__uint_as_float(u) * 512.0F));
+    return sign | m;
+}
+
+__device__ __forceinline__ float e4m3_value(uint32_t b)
+{
+    const uint32_t e = (b >> 3) & 0xfU;
+    const uint32_t m = b & 7U;
+    const float mag = e == 0 ? static_cast<float>(m) * 0.001953125F  // m * 2^-9
+                             : ldexpf(1.0F + static_cast<float>(m) * 0.125F, static_cast<int>(e) - 7);
+    return (b & 0x80U) ? -mag : mag;
+}
+
+__device__ __forceinline__ float block_max_f32(float v, float* scratch)
+{
+    for (int off = 32; off >= 1; off /= 2) v = fmaxf(v, __shfl_down(v, off));
+    const int lane = threadIdx.x % 64;
+    const int wave = threadIdx.x / 64;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    float m = 0.0F;
+    for (int w = 0; w < static_cast<int>(blockDim.x / 64); ++w) m = fmaxf(m, scratch[w]);
+    return m;
+}
+
+// One block of 256 threads per row: max |x| -> scale exponent -> bytes.
+__global__ void __launch_bounds__(256)
+to_fp8_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows, uint32_t dim,
+                   uint8_t* __restrict__ dst, uint64_t ld_dst, RowStats* __restrict__ stats)
+{
+    __shared__ double scratch[8];
+    __shared__ float fscratch[8];
+    const uint32_t row = blockIdx.x;
+    if (row >= rows) return;
+    const float* s = src + static_cast<uint64_t>(row) * ld_src;
+    uint8_t* o = dst + static_cast<uint64_t>(row) * ld_dst;
+    float mx = 0.0F;
+    for (uint32_t c = threadIdx.x; c < dim; c += blockDim.x) mx = fmaxf(mx, fabsf(s[c]));
+    mx = block_max_f32(mx, fscratch);
+    // Smallest c with 448 * 2^c >= mx: mx = f 2^e, f in [0.5, 1), 448 = 0.875 2^9.
+    int c_exp = 0;
+    if (mx > 0.0F) {
+        int e = 0;
+        const float f = frexpf(mx, &e);
+        c_exp = f <= 0.875F ? e - 9 : e - 8;
+        c_exp = c_exp < -126 ? -126 : c_exp;
+    }
+    const float scale = ldexpf(1.0F, c_exp);
+    double n2 = 0.0, n4 = 0.0, h2 = 0.0, h4 = 0.0, e4 = 0.0;
+    for (uint32_t c = threadIdx.x; c < ld_dst; c += blockDim.x) {
+        const float x = c < dim ? s[c] : 0.0F;
+        const uint32_t b = e4m3_rne(ldexpf(x, -c_exp));
+        o[c] = static_cast<uint8_t>(b);
+        const double xd = x;
+        const double hd = static_cast<double>(e4m3_value(b)) * static_cast<double>(scale);
+        const double ed = xd - hd;
+        n2 += xd * xd;
+        n4 += (xd * xd) * (xd * xd);
+        h2 += hd * hd;
+        h4 += (hd * hd) * (hd * hd);
+        e4 += (ed * ed) * (ed * ed);
+    }
+    n2 = block_sum_f64(n2, scratch);
+    n4 = block_sum_f64(n4, scratch);
+    h2 = block_sum_f64(h2, scratch);
+    h4 = block_sum_f64(h4, scratch);
+    e4 = block_sum_f64(e4, scratch);
+    if (threadIdx.x == 0) {
+        RowStats st{};
+        st.norm2 = static_cast<float>(n2);
+        st.l4 = static_cast<float>(sqrt(sqrt(n4))) * 1.0001F;
+        st.l4_hat = static_cast<float>(sqrt(sqrt(h4))) * 1.0001F;
+        st.l2_hat = static_cast<float>(sqrt(h2)) * 1.0001F;
+        st.d4 = static_cast<float>(sqrt(sqrt(e4))) * 1.0001F;
+        st.scale = scale;
+        stats[row] = st;
+    }
+}
+
+typedef int intx8 __attribute__((ext_vector_type(8)));
+
+// 256 x 256 outputs per block, 4 x 4 waves of 2 x 2 32x32 tiles, as the
+// BF16 kernel; a K step is 128 e4m3 bytes per row (two MFMA K=64 steps).
+// Operand fragments: lane holds row (lane % 32) and the 32 K values
+// 32*(lane/32) .. +31 of each 64-wide step (the same assignment for both
+// operands, which is what the MFMA's dot product needs).
+__global__ void __launch_bounds__(1024)
+gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const uint8_t* __restrict__ V,
+                uint64_t ldv, uint32_t N, uint32_t K, const RowStats* __restrict__ qs,
+                const RowStats* __restrict__ vs, float* __restrict__ out, uint64_t ldo, uint32_t tiles_m,
+                uint32_t tiles_n, uint32_t group_m)
+{
+    constexpr int WN = 4, NT = 1024, BM = 256, BN = 256;
+    constexpr int CA = BM * 8 / NT;  // 16-byte chunks per thread per K step
+    constexpr int CB = BN * 8 / NT;
+    __shared__ __attribute__((aligned(16))) uint8_t As[2 * BM * kLdsRow8];
+    __shared__ __attribute__((aligned(16))) uint8_t Bs[2 * BN * kLdsRow8];
+
+    const uint32_t L = xcd_logical(blockIdx.x, tiles_m * tiles_n);
+    const uint32_t span = group_m * tiles_n;
+    const uint32_t gid = L / span;
+    const uint32_t first_m = gid * group_m;
+    const uint32_t gsize = min(group_m, tiles_m - first_m);
+    const uint32_t within = L - gid * span;
+    const uint32_t m0 = (first_m + within % gsize) * BM;
+    const uint32_t n0 = (within / gsize) * BN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid % 64;
+    const int wave = tid / 64;
+    const int wm = (wave / WN) * 64;
+    const int wn = (wave % WN) * 64;
+
+    uint4 ra[CA], rb[CB];
+    const uint8_t* qbase = Q + static_cast<uint64_t>(m0 + tid / 8) * ldq + (tid % 8) * 16;
+    const uint8_t* vbase = V + static_cast<uint64_t>(n0 + tid / 8) * ldv + (tid % 8) * 16;
+    const uint64_t qstep = static_cast<uint64_t>(NT / 8) * ldq;
+    const uint64_t vstep = static_cast<uint64_t>(NT / 8) * ldv;
+    const int lds0 = (tid / 8) * kLdsRow8 + (tid % 8) * 16;
+    constexpr int kLdsStep = (NT / 8) * kLdsRow8;
+    const int a_off = (wm + lane % 32) * kLdsRow8 + (lane / 32) * 32;
+    const int b_off = (wn + lane % 32) * kLdsRow8 + (lane / 32) * 32;
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
+
+    auto frag = [](const uint8_t* p) {
+        const uint4 lo = *reinterpret_cast<const uint4*>(p);
+        const uint4 hi = *reinterpret_cast<const uint4*>(p + 16);
+        intx8 f;
+        f[0] = static_cast<int>(lo.x);
+        f[1] = static_cast<int>(lo.y);
+        f[2] = static_cast<int>(lo.z);
+        f[3] = static_cast<int>(lo.w);
+        f[4] = static_cast<int>(hi.x);
+        f[5] = static_cast<int>(hi.y);
+        f[6] = static_cast<int>(hi.z);
+        f[7] = static_cast<int>(hi.w);
+        return f;
+    };
+#define UHANN_FP8_LOAD(k0)                                                                    \
+    _Pragma("unroll") for (int c = 0; c < CA; ++c) ra[c] =                                    \
+        *reinterpret_cast<const uint4*>(qbase + c * qstep + (k0));                            \
+    _Pragma("unroll") for (int c = 0; c < CB; ++c) rb[c] =                                    \
+        *reinterpret_cast<const uint4*>(vbase + c * vstep + (k0));                            \
+    __builtin_amdgcn_sched_barrier(0);
+#define UHANN_FP8_STORE(buf)                                                                  \
+    _Pragma("unroll") for (int c = 0; c < CA; ++c)                                            \
+        *reinterpret_cast<uint4*>(As + (buf) * BM * kLdsRow8 + lds0 + c * kLdsStep) = ra[c];  \
+    _Pragma("unroll") for (int c = 0; c < CB; ++c)                                            \
+        *reinterpret_cast<uint4*>(Bs + (buf) * BN * kLdsRow8 + lds0 + c * kLdsStep) = rb[c];
+#define UHANN_FP8_COMPUTE(buf)                                                                \
+    _Pragma("unroll") for (int kk = 0; kk < kBK8; kk += 64)                                   \
+    {                                                                                         \
+        const uint8_t* ab = As + (buf) * BM * kLdsRow8 + a_off + kk;                          \
+        const uint8_t* bb = Bs + (buf) * BN * kLdsRow8 + b_off + kk;                          \
+        const intx8 a0 = frag(ab);                                                            \
+        const intx8 a1 = frag(ab + 32 * kLdsRow8);                                            \
+        const intx8 b0 = frag(bb);                                                            \
+        const intx8 b1 = frag(bb + 32 * kLdsRow8);                                            \
+        acc[0][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a0, b0, acc[0][0], 0, 0, 0, 127, 0, 127); \
+        acc[0][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a0, b1, acc[0][1], 0, 0, 0, 127, 0, 127); \
+        acc[1][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, b0, acc[1][0], 0, 0, 0, 127, 0, 127); \
+        acc[1][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, b1, acc[1][1], 0, 0, 0, 127, 0, 127); \
+    }
+
+    UHANN_FP8_LOAD(0)
+    UHANN_FP8_STORE(0)
+    __syncthreads();
+    int cur = 0;
+    for (uint32_t k0 = kBK8; k0 < K; k0 += kBK8) {
+        UHANN_FP8_LOAD(k0)
+        UHANN_FP8_COMPUTE(cur)
+        UHANN_FP8_STORE(cur ^ 1)
+        __syncthreads();
+        cur ^= 1;
+    }
+    UHANN_FP8_COMPUTE(cur)
+#undef UHANN_FP8_LOAD
+#undef UHANN_FP8_STORE
+#undef UHANN_FP8_COMPUTE
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const uint32_t n = n0 + wn + j * 32 + (lane % 32);
+            if (n >= N) continue;
+            const RowStats vn = vs[n];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t m = m0 + wm + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (m < M) {
+                    const float base = qs[m].norm2 + vn.norm2;
+                    const float sc = qs[m].scale * vn.scale;
+                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * (sc * acc[i][j][v]);
+                }
+            }
+        }
+    }
+}
+
+// Top-kShortlist selection per query (64-bit keys (ordered estimate << 32 |
+// row): per-thread sorted lists, then an 8-pass radix select of the
+// kShortlist-th smallest key over their union), exact re-rank, and the
+// certification pass.
+__global__ void __launch_bounds__(kSelThreads)
+select_topk_rerank_kernel(SelectArgs a)
+{
+    __shared__ unsigned int s_hist[256];
+    __shared__ unsigned long long s_prefix;
+    __shared__ uint32_t s_need;
+    __shared__ uint32_t s_count;
+    __shared__ uint32_t s_cand[kShortlist];
+    __shared__ float s_dist[kShortlist];
+    __shared__ float s_group[8][kRerankRows];
+    __shared__ double s_scratch[8];
+    __shared__ uint32_t s_best;
+    __shared__ float s_best_d;
+    __shared__ int s_uncertified;
+
+    const uint32_t q = blockIdx.x;
+    if (q >= a.nq) return;
+    const int t = threadIdx.x;
+    const float* arow = a.approx + static_cast<uint64_t>(q) * a.ld_approx;
+
+    unsigned long long top[kShortlist];
+#pragma unroll
+    for (int i = 0; i < kShortlist; ++i) top[i] = ~0ULL;
+    for (uint32_t r = t; r < a.rows; r += kSelThreads) {
+        const unsigned long long key = (static_cast<unsigned long long>(ordered(arow[r])) << 32) | r;
+        if (key < top[kShortlist - 1]) {
+            // insert into the ascending list (the last entry drops out)
+            unsigned long long cur = key;
+#pragma unroll
+            for (int i = 0; i < kShortlist; ++i) {
+                const unsigned long long lo = cur < top[i] ? cur : top[i];
+                const unsigned long long hi = cur < top[i] ? top[i] : cur;
+                top[i] = lo;
+                cur = hi;
+            }
+        }
+    }
+    const uint32_t need0 = min(static_cast<uint32_t>(kShortlist), a.rows);
+    if (t == 0) {
+        s_prefix = 0ULL;
+        s_need = need0;
+        s_count = 0;
+        s_uncertified = 0;
+    }
+    unsigned long long mask = 0ULL;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
+        s_hist[t] = 0U;
+        __syncthreads();
+        const unsigned long long prefix = s_prefix;
+#pragma unroll
+        for (int i = 0; i < kShortlist; ++i) {
+            if (top[i] != ~0ULL && (top[i] & mask) == prefix) {
+                atomicAdd(&s_hist[(top[i] >> shift) & 255ULL], 1U);
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            uint32_t need = s_need;
+            uint32_t bin = 0;
+            for (; bin < 255U; ++bin) {
+                if (s_hist[bin] >= need) break;
+                need -= s_hist[bin];
+            }
+            s_need = need;
+            s_prefix = prefix | (static_cast<unsigned long long>(bin) << shift);
+        }
+        mask |= 255ULL << shift;
+        __syncthreads();
+    }
+    const unsigned long long threshold = s_prefix;  // the need0-th smallest key
+#pragma unroll
+    for (int i = 0; i < kShortlist; ++i) {
+        if (top[i] <= threshold) {
+            const uint32_t slot = atomicAdd(&s_count, 1U);
+            if (slot < kShortlist) s_cand[slot] = static_cast<uint32_t>(top[i] & 0xffffffffULL);
+        }
+    }
+    __syncthreads();
+    const uint32_t count = min(s_count, static_cast<uint32_t>(kShortlist));
+
+    uint32_t best;
+    float best_d;
+    rerank_pick(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
+
+    // Certified when no row outside the shortlist can be closer: its
+    // estimate minus its error bound exceeds the answer's distance (widened
+    // by the binary32 band).
+    const RowStats qs = a.qstats[q];
+    const float upper = static_cast<float>(static_cast<double>(best_d) * a.fp32_band);
+    int bad = 0;
+    for (uint32_t r = t; r < a.rows; r += kSelThreads) {
+        const float e = arow[r];
+        const unsigned long long key = (static_cast<unsigned long long>(ordered(e)) << 32) | r;
+        if (key > threshold && !(e - pair_tau(qs, a.vstats[r], a.acc_coef) > upper)) bad = 1;
+    }
+    if (bad) atomicOr(&s_uncertified, 1);
+    __syncthreads();
+    if (t == 0) {
+        a.out_idx[q] = best;
+        a.out_dist[q] = best_d;
+        if (a.stats != nullptr) {
+            atomicAdd(&a.stats->candidates, count);
+            if (s_uncertified == 0) atomicAdd(&a.stats->certified, 1U);
+        }
     }
 }
 
@@ -543,6 +899,36 @@ void select_rerank(const SelectArgs& args, hipStream_t stream)
     select_rerank_kernel<<<args.nq, kSelThreads, 0, stream>>>(args);
     check_launch("candidate selection / re-rank kernel");
 }
+
+void to_fp8_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, uint8_t* dst, uint64_t ld_dst,
+                 RowStats* stats, hipStream_t stream)
+{
+    if (rows == 0) return;
+    to_fp8_rows_kernel<<<rows, 256, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    check_launch("fp8 conversion kernel");
+}
+
+void gram_fp8(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uint64_t ldv, uint32_t n, uint32_t k,
+              const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo, hipStream_t stream)
+{
+    if (m == 0 || n == 0) return;
+    if (k % kBK8 != 0) throw std::invalid_argument("FP8 Gram K must be a multiple of 128");
+    if (ldq % 16 != 0 || ldv % 16 != 0) throw std::invalid_argument("FP8 Gram rows must be 16-byte aligned");
+    const uint32_t tiles_m = (m + 255U) / 256U;
+    const uint32_t tiles_n = (n + 255U) / 256U;
+    gram_fp8_kernel<<<tiles_m * tiles_n, 1024, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo, tiles_m,
+                                                           tiles_n, 8U);
+    check_launch("fp8 Gram kernel");
+}
+
+void select_topk_rerank(const SelectArgs& args, hipStream_t stream)
+{
+    if (args.nq == 0) return;
+    select_topk_rerank_kernel<<<args.nq, kSelThreads, 0, stream>>>(args);
+    check_launch("shortlist re-rank kernel");
+}
+
+uint32_t gram_fp8_k_padding() { return kBK8; }
 
 uint32_t gram_k_padding() { return kBK; }
 int gram_tile_count() { return 8; }

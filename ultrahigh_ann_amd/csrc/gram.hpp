@@ -1,5 +1,6 @@
-// Launch interfaces of the mixed-precision (BF16 Gram + exact re-rank)
-// kernels in gram.hip.
+// Launch interfaces of the mixed-precision kernels in gram.hip: BF16 and
+// per-row-scaled FP8 Gram estimates, the certified BF16 candidate re-rank and
+// the FP8 top-k shortlist re-rank.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -8,24 +9,27 @@
 
 namespace uhann::kern {
 
-// Per-row statistics used by the Gram estimate and its error bound.
+// Per-row statistics used by the Gram estimate and its error bound; x^ is
+// the row as the Gram kernel sees it (bf16, or scale * e4m3).
 struct RowStats {
     float norm2;   // ||x||^2 of the binary32 row (binary64 sum, rounded)
     float l4;      // ||x||_4
-    float l4_hat;  // ||bf16(x)||_4
-    float l2_hat;  // ||bf16(x)||_2
-    float d4;      // ||x - bf16(x)||_4, the row's actual rounding error
-    float pad[3];
+    float l4_hat;  // ||x^||_4
+    float l2_hat;  // ||x^||_2
+    float d4;      // ||x - x^||_4, the row's actual rounding error
+    float scale;   // FP8: power-of-two row scale (x^ = scale * e4m3); BF16: 1
+    float pad[2];
 };
 
 struct MixedStats {
     unsigned int candidates;     // sum over queries of the candidate-set size
     unsigned int fp64_reranked;  // queries decided in binary64
     unsigned int escalated;      // queries sent to the exact direct scan
-    unsigned int pad;
+    unsigned int certified;      // FP8 shortlist: queries whose answer the error bound proves
 };
 
 constexpr int kMaxCandidates = 64;
+constexpr int kShortlist = 16;  // FP8 strategy: candidates re-ranked per query
 
 struct SelectArgs {
     uint32_t nq = 0;
@@ -35,7 +39,7 @@ struct SelectArgs {
     uint64_t ld_approx = 0;
     const RowStats* qstats = nullptr;
     const RowStats* vstats = nullptr;
-    float acc_steps_sqrt = 0.0F;    // sqrt(ceil(K/16))
+    float acc_coef = 0.0F;          // accumulation term: acc_coef u_32 ||q^||_2 ||v^||_2
     const float* queries = nullptr; // binary32 queries, row stride ld_q
     uint64_t ld_q = 0;
     const float* refs = nullptr;    // binary32 references, row stride ld_ref
@@ -59,6 +63,23 @@ void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint6
                uint32_t k, const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo,
                int tile, hipStream_t stream);
 void select_rerank(const SelectArgs& args, hipStream_t stream);
+
+// binary32 rows -> e4m3 bytes (round to nearest even) with a power-of-two
+// row scale 2^ceil(log2(max|x| / 448)) (1 for a zero row), columns padded
+// with zeros to ld_dst bytes, plus row statistics of scale * e4m3.
+void to_fp8_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, uint8_t* dst,
+                 uint64_t ld_dst, RowStats* stats, hipStream_t stream);
+// out[m][n] = ||q||^2 + ||v||^2 - 2 s_q s_v <q8, v8> with the inner product
+// on v_mfma_scale_f32_32x32x64_f8f6f4 (binary32 accumulation); k a multiple
+// of gram_fp8_k_padding(), q and v padded to whole tiles like gram_bf16.
+void gram_fp8(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uint64_t ldv, uint32_t n, uint32_t k,
+              const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo, hipStream_t stream);
+// Per query: the kShortlist rows with the smallest estimates (ties to the
+// lower row), re-ranked exactly in the binary32 reference order, binary64
+// for near ties; stats->certified counts queries where every other row's
+// estimate minus its error bound exceeds the answer's distance.
+void select_topk_rerank(const SelectArgs& args, hipStream_t stream);
+uint32_t gram_fp8_k_padding();
 uint32_t gram_k_padding();
 int gram_tile_count();
 uint32_t gram_m_tile();

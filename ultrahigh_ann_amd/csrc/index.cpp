@@ -114,6 +114,8 @@ void index_launch(const uhann_index& ix, Workspace& ws, const float* queries, ui
                         nullptr);
         } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
             mixed_launch(*ix.scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
+        } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {
+            fp8_launch(*ix.scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
         } else {
             throw std::invalid_argument("unknown query strategy");
         }
@@ -129,6 +131,8 @@ void index_launch(const uhann_index& ix, Workspace& ws, const float* queries, ui
                         nullptr);
         } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
             mixed_launch(*ix.scan, ws, gathered, ix.k_ld, count, out_idx, out_dist, stream);
+        } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {
+            fp8_launch(*ix.scan, ws, gathered, ix.k_ld, count, out_idx, out_dist, stream);
         } else {
             throw std::invalid_argument("unknown query strategy");
         }
@@ -365,11 +369,13 @@ int uhann_index_enable_mixed(uhann_index* ix, int strategy)
 {
     return guarded([&] {
         if (ix == nullptr) throw std::invalid_argument("index handle must not be null");
-        if (strategy != UHANN_STRATEGY_GRAM_BF16) throw std::invalid_argument("unknown mixed strategy");
+        if (strategy != UHANN_STRATEGY_GRAM_BF16 && strategy != UHANN_STRATEGY_GRAM_FP8) {
+            throw std::invalid_argument("unknown mixed strategy");
+        }
         if (ix->kind == UHANN_KIND_HIERARCHICAL) {
             throw std::invalid_argument("the mixed strategy applies to exact and flat indexes");
         }
-        ix->scan->enable_mixed();
+        ix->scan->enable_mixed(strategy);
     });
 }
 

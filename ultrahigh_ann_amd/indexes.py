@@ -23,6 +23,7 @@ METRIC_L1 = 1
 STRATEGY_DIRECT = 0
 STRATEGY_GRAM_BF16 = 1
 STRATEGY_PROJECTION_BF16 = 2
+STRATEGY_GRAM_FP8 = 3
 KIND_EXACT, KIND_FLAT, KIND_HIERARCHICAL = 0, 1, 2
 
 SAMPLE_DTYPE = np.dtype([("source_column", "<u8"), ("multiplicity", "<u8"), ("inverse_probability", "<f8")])
@@ -125,6 +126,12 @@ class IndexWorkspace:
     def status(self) -> None:
         """Raise the data error left by asynchronous batches, if any."""
         check(_native.lib().uhann_workspace_status(self._h))
+
+    def shortlist_stats(self) -> dict:
+        """Certified queries of the last GRAM_FP8 batch."""
+        c = c_uint64()
+        check(_native.lib().uhann_workspace_shortlist_stats(self._h, ctypes.byref(c)))
+        return {"certified": c.value}
 
     def mixed_stats(self) -> dict:
         c, f, e = c_uint64(), c_uint64(), c_uint64()
