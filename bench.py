@@ -212,6 +212,43 @@ def serving(uh, torch, refs_np, queries, args, n_indexes=64):
     return out
 
 
+def centroids_r33(uh, torch, args):
+    """Config of the reference's CPU study: flat L2, 33 class centroids (mean
+    of 90 samples each), d = 60660, 3013 held-out queries, T = 8 over the
+    sampling seeds 7, 20, 42, 1312, 2026, against exact full-d L2 (the FP32
+    direct scan, which returns the reference's CUDA bits): throughput,
+    agreement with exact and relative distance excess (binary64 distances)."""
+    from ultrahigh_ann_amd import synthetic
+
+    cents, q, _ = synthetic.centroid_workload(d=args.dim, seed=args.seed, device="cuda")
+    refs_np = cents.cpu().numpy()
+    B = q.shape[0]
+    oi = torch.zeros(B, dtype=torch.int64, device="cuda")
+    od = torch.zeros(B, dtype=torch.float32, device="cuda")
+    ex = uh.HipExactIndex(refs_np)
+    ex_ws = ex.make_query_workspace(B)
+    wall, ms = timed(lambda: ex.query_device_async(q, B, ex_ws, oi, od), 3, 1, torch)
+    exact_idx = oi.cpu().numpy().astype(np.uint64)
+    out = {"references": 33, "queries": B, "dim": args.dim,
+           "exact_fp32": {"queries_per_s": round(B / float(np.median(ms) * 1e-3), 1)}}
+    probs = uh.importance_probabilities_device(uh.METRIC_L2, refs_np)
+    per_seed = {}
+    for seed in (7, 20, 42, 1312, 2026):
+        index = uh.HipFlatL2AnnIndex(refs_np, probs, 8, uh.Rng(seed))
+        ws = index.make_query_workspace(B)
+        wall, ms = timed(lambda: index.query_device_async(q, B, ws, oi, od), 3, 1, torch)
+        got = oi.cpu().numpy().astype(np.uint64)
+        row = {"unique_coordinates": index.space_usage()["unique_query_coordinates"],
+               "queries_per_s": round(B / float(np.median(ms) * 1e-3), 1),
+               "us_per_query": round(float(np.median(ms)) * 1e3 / B, 4)}
+        row.update(quality(torch, cents, q, got, exact_idx, False))
+        per_seed[str(seed)] = row
+    out["flat_T8"] = per_seed
+    out["flat_T8_mean"] = {k: round(float(np.mean([v[k] for v in per_seed.values()])), 5)
+                           for k in ("queries_per_s", "agreement", "mean_excess_pct", "over_10pct")}
+    return out
+
+
 def mixed_large(uh, torch, args, r=65536, B=8192, sample=16):
     """Mixed-precision exact L2 at r = 65536 synthetic points (FP32 copy
     15.9 GB, BF16 copy 8.0 GB), d = 60660, batch 8192: BF16 Gram + certified
@@ -310,9 +347,20 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
             wall, ms = timed(lambda: index.query_device_async(queries, b, ws, oi, od), 5, 2, torch)
             t = float(np.median(ms)) * 1e-3
             bytes_alg = 4.0 * r * d + 4.0 * b * d + 12.0 * b
+            # the scan kernel alone (HIP events around it, without the key
+            # fill / finalise launches of the call)
+            ws.set_timing(True)
+            kms = []
+            for _ in range(5):
+                index.query_device_async(queries, b, ws, oi, od)
+                kms.append(ws.stage_times()[0])
+            ws.set_timing(False)
+            tk = float(np.median(kms)) * 1e-3
             rows[f"batch_{b}"] = {"queries_per_s": round(b / t, 1), "ms": round(t * 1e3, 4),
                                   "hbm_gbps": round(bytes_alg / t / 1e9, 1),
-                                  "hbm_frac": round(bytes_alg / t / 8.0e12, 4)}
+                                  "hbm_frac": round(bytes_alg / t / 8.0e12, 4),
+                                  "kernel_ms": round(tk * 1e3, 4),
+                                  "kernel_hbm_frac": round(bytes_alg / tk / 8.0e12, 4)}
         return rows
 
     probs = {}
@@ -379,6 +427,7 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
 
     add("serving_64_indexes", lambda: serving(uh, torch, refs_np, queries, args))
     add("exact_l2_r65536_b8192_mixed", lambda: mixed_large(uh, torch, args))
+    add("flat_l2_r33_centroids", lambda: centroids_r33(uh, torch, args))
     add("exact_l1_fp32_direct", exact_l1)
     add("exact_l2_streaming", streaming)
     add("flat_l2", lambda: flat(uh.METRIC_L2))

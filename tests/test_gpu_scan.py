@@ -197,3 +197,24 @@ def test_autotune_keeps_bits(uh, oracle_mod, d):
         idx, dist = index.query_device_batch(dq[:n], n, ws)
         np.testing.assert_array_equal(idx, exp_idx[:n])
         np.testing.assert_array_equal(dist.view(np.uint32), exp_dist[:n].view(np.uint32))
+
+
+def test_direct_kernel_timing_keeps_results(uh, oracle_mod):
+    """Stage timing around the scan kernel (events on the workspace stream)
+    reports a positive kernel time and leaves the results unchanged."""
+    import torch
+
+    refs, q = _data(21, 90, 5000, 3)
+    index = uh.HipExactIndex(refs)
+    ws = index.make_query_workspace(3)
+    dq = torch.from_numpy(q).cuda()
+    plain = index.query_device_batch(dq, 3, ws)
+    ws.set_timing(True)
+    timed_run = index.query_device_batch(dq, 3, ws)
+    times = ws.stage_times()
+    ws.set_timing(False)
+    np.testing.assert_array_equal(plain[0], timed_run[0])
+    np.testing.assert_array_equal(plain[1].view(np.uint32), timed_run[1].view(np.uint32))
+    assert times[0] > 0.0 and times[1:] == [0.0, 0.0, 0.0]
+    exp_idx, _ = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q))
+    np.testing.assert_array_equal(plain[0], exp_idx)

@@ -265,7 +265,7 @@ bool stream_refs_enabled();
 
 void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t count,
                  unsigned long long* keys, uint64_t* out_idx, float* out_dist,
-                 hipStream_t stream, bool queries_padded, const kern::ScanTile* forced_tile)
+                 hipStream_t stream, bool queries_padded, const kern::ScanTile* forced_tile, Workspace* timed)
 {
     kern::ScanArgs a;
     a.refs = scan.refs.data();
@@ -296,10 +296,18 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     if (tile.rpt == 0) {
         if (!pairs_ok) throw std::invalid_argument("pair scan needs rows of at most 4096 coordinates");
         kern::fill_keys(keys, a.nq, stream);
+        if (timed != nullptr) timed->mark(0, stream);
         kern::scan_pairs(scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1,
                          /*weight_fma=*/true, scan.refs_t.data(), scan.rows, a.rows, queries, ld_q, a.nq,
                          a.weights, a.dim, keys, tile.tq, stream);
+        if (timed != nullptr) timed->mark(1, stream);
         kern::finalize_keys(keys, a.nq, out_idx, out_dist, stream);
+        if (timed != nullptr && timed->timing) {
+            check_hip(hipEventSynchronize(timed->events[1]), "HIP event wait failed");
+            check_hip(hipEventElapsedTime(&timed->stage_ms[0], timed->events[0], timed->events[1]),
+                      "cannot read HIP event time");
+            for (int i = 1; i < Workspace::kStages; ++i) timed->stage_ms[i] = 0.0F;
+        }
         return;
     }
     if (tile.rpt > 1 && !aligned4) {
@@ -314,9 +322,18 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
         a.ld_q = 0;
     }
     kern::fill_keys(keys, a.nq, stream);
+    if (timed != nullptr) timed->mark(0, stream);
     kern::scan_direct(scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1,
                       /*weight_fma=*/true, a, tile, stream);
+    if (timed != nullptr) timed->mark(1, stream);
     kern::finalize_keys(keys, a.nq, out_idx, out_dist, stream);
+    if (timed != nullptr && timed->timing) {
+        // Stage 0 = the scan kernel alone (without the key fill / finalise launches).
+        check_hip(hipEventSynchronize(timed->events[1]), "HIP event wait failed");
+        check_hip(hipEventElapsedTime(&timed->stage_ms[0], timed->events[0], timed->events[1]),
+                  "cannot read HIP event time");
+        for (int i = 1; i < Workspace::kStages; ++i) timed->stage_ms[i] = 0.0F;
+    }
 }
 
 
@@ -566,7 +583,7 @@ void run_strategy(const Scan& scan, Workspace& ws, const float* queries, uint64_
 {
     if (strategy == UHANN_STRATEGY_DIRECT) {
         scan_launch(scan, queries, ld_q, count, ws.keys.data(), out_idx, out_dist, stream, padded,
-                    nullptr);
+                    nullptr, &ws);
     } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
         mixed_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
     } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {

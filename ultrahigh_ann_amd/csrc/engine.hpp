@@ -129,9 +129,11 @@ void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_
 // fill keys -> direct scan -> finalize, all on `stream`. queries_padded
 // says every query row is readable up to the next multiple of 4 floats past
 // dim (workspace-staged rows are). forced_tile (or null) overrides the tile.
+// With `timed` (a workspace whose timing is on) stage 0 receives the scan
+// kernel's own duration.
 void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t count,
                  unsigned long long* keys, uint64_t* out_idx, float* out_dist, hipStream_t stream,
-                 bool queries_padded, const kern::ScanTile* forced_tile);
+                 bool queries_padded, const kern::ScanTile* forced_tile, Workspace* timed = nullptr);
 
 // Mixed-precision L2 query (BF16 Gram -> candidate set -> exact re-rank ->
 // binary64 near ties -> direct scan for escalated queries). Synchronises

@@ -300,14 +300,17 @@ constexpr int kRerankRows = 4;
 constexpr int kKeepLo = 32;  // approx - tau values kept in registers per thread
 
 // One block per query. See the file comment for the algorithm.
-// Exact re-rank of `count` candidate rows s_cand of query q: binary32
+// Exact re-rank of `count` candidate rows s_cand of query q (ROWS per pass
+// over the query row): binary32
 // distances in the reference's order (thread t is chain t) into s_dist, the
 // lowest (distance, row), then the binary64 minimum among candidates inside
 // the binary32 error band. Uses the whole block.
+template <int ROWS>
 __device__ void rerank_pick(const SelectArgs& a, uint32_t q, const uint32_t* s_cand, uint32_t count, float* s_dist,
-                            float (*s_group)[kRerankRows], double* s_scratch, uint32_t* s_best, float* s_best_d,
+                            float (*s_group)[ROWS], double* s_scratch, uint32_t* s_best, float* s_best_d,
                             uint32_t& best, float& best_d)
 {
+    constexpr int kRerankRows = ROWS;  // candidate rows re-ranked per pass over the query row
     const int t = threadIdx.x;
     // Exact binary32 re-rank in the reference's order: thread t is chain t.
     const float* qrow = a.queries + static_cast<uint64_t>(q) * a.ld_q;
@@ -490,7 +493,7 @@ select_rerank_kernel(SelectArgs a)
 
     uint32_t best;
     float best_d;
-    rerank_pick(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
+    rerank_pick<kRerankRows>(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
     if (t == 0) {
         a.out_idx[q] = best;
         a.out_dist[q] = best_d;
@@ -555,8 +558,15 @@ to_fp8_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows
     if (row >= rows) return;
     const float* s = src + static_cast<uint64_t>(row) * ld_src;
     uint8_t* o = dst + static_cast<uint64_t>(row) * ld_dst;
+    // 16-byte aligned rows (ld_src % 4 == 0) are read four values at a time.
+    const bool vec = ld_src % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15U) == 0;
+    const uint32_t full = vec ? dim / 4 * 4 : 0;
     float mx = 0.0F;
-    for (uint32_t c = threadIdx.x; c < dim; c += blockDim.x) mx = fmaxf(mx, fabsf(s[c]));
+    for (uint32_t c = threadIdx.x * 4; c < full; c += blockDim.x * 4) {
+        const float4 x = *reinterpret_cast<const float4*>(s + c);
+        mx = fmaxf(fmaxf(mx, fmaxf(fabsf(x.x), fabsf(x.y))), fmaxf(fabsf(x.z), fabsf(x.w)));
+    }
+    for (uint32_t c = full + threadIdx.x; c < dim; c += blockDim.x) mx = fmaxf(mx, fabsf(s[c]));
     mx = block_max_f32(mx, fscratch);
     // Smallest c with 448 * 2^c >= mx: mx = f 2^e, f in [0.5, 1), 448 = 0.875 2^9.
     int c_exp = 0;
@@ -568,10 +578,8 @@ to_fp8_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows
     }
     const float scale = ldexpf(1.0F, c_exp);
     double n2 = 0.0, n4 = 0.0, h2 = 0.0, h4 = 0.0, e4 = 0.0;
-    for (uint32_t c = threadIdx.x; c < ld_dst; c += blockDim.x) {
-        const float x = c < dim ? s[c] : 0.0F;
+    auto one = [&](float x) {
         const uint32_t b = e4m3_rne(ldexpf(x, -c_exp));
-        o[c] = static_cast<uint8_t>(b);
         const double xd = x;
         const double hd = static_cast<double>(e4m3_value(b)) * static_cast<double>(scale);
         const double ed = xd - hd;
@@ -580,6 +588,16 @@ to_fp8_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows
         h2 += hd * hd;
         h4 += (hd * hd) * (hd * hd);
         e4 += (ed * ed) * (ed * ed);
+        return b;
+    };
+    // ld_dst is a multiple of 128, so four-byte groups never straddle a row.
+    for (uint32_t c = threadIdx.x * 4; c < full; c += blockDim.x * 4) {
+        const float4 x = *reinterpret_cast<const float4*>(s + c);
+        const uint32_t packed = one(x.x) | (one(x.y) << 8) | (one(x.z) << 16) | (one(x.w) << 24);
+        *reinterpret_cast<uint32_t*>(o + c) = packed;
+    }
+    for (uint32_t c = full + threadIdx.x; c < ld_dst; c += blockDim.x) {
+        o[c] = static_cast<uint8_t>(one(c < dim ? s[c] : 0.0F));
     }
     n2 = block_sum_f64(n2, scratch);
     n4 = block_sum_f64(n4, scratch);
@@ -739,7 +757,7 @@ select_topk_rerank_kernel(SelectArgs a)
     __shared__ uint32_t s_count;
     __shared__ uint32_t s_cand[kShortlist];
     __shared__ float s_dist[kShortlist];
-    __shared__ float s_group[8][kRerankRows];
+    __shared__ float s_group[8][kShortlist];
     __shared__ double s_scratch[8];
     __shared__ uint32_t s_best;
     __shared__ float s_best_d;
@@ -813,7 +831,7 @@ select_topk_rerank_kernel(SelectArgs a)
 
     uint32_t best;
     float best_d;
-    rerank_pick(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
+    rerank_pick<kShortlist>(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
 
     // Certified when no row outside the shortlist can be closer: its
     // estimate minus its error bound exceeds the answer's distance (widened

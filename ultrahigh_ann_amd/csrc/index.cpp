@@ -111,7 +111,7 @@ void index_launch(const uhann_index& ix, Workspace& ws, const float* queries, ui
     if (ix.kind == UHANN_KIND_EXACT) {
         if (strategy == UHANN_STRATEGY_DIRECT) {
             scan_launch(*ix.scan, queries, ld_q, count, ws.keys.data(), out_idx, out_dist, stream, padded,
-                        nullptr);
+                        nullptr, &ws);
         } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
             mixed_launch(*ix.scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
         } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {

@@ -127,6 +127,18 @@ class IndexWorkspace:
         """Raise the data error left by asynchronous batches, if any."""
         check(_native.lib().uhann_workspace_status(self._h))
 
+    def set_timing(self, enable: bool) -> None:
+        """HIP-event stage timing of later calls on this workspace."""
+        check(_native.lib().uhann_workspace_set_timing(self._h, int(bool(enable))))
+
+    def stage_times(self) -> list:
+        """Stage times (ms) of the last timed call: the direct scan kernel alone
+        in [0]; for the mixed strategies conversion, Gram, selection + re-rank,
+        escalation."""
+        ms = (ctypes.c_float * 4)()
+        check(_native.lib().uhann_workspace_stage_times(self._h, ms, 4))
+        return list(ms)
+
     def shortlist_stats(self) -> dict:
         """Certified queries of the last GRAM_FP8 batch."""
         c = c_uint64()

@@ -45,3 +45,28 @@ def log_expression(r: int, b: int, d: int = 60660, seed: int = 2026, device="cud
     refs = samples(prof, r, gen)
     queries = samples(prof, b, gen)
     return refs, queries
+
+
+def centroid_workload(d: int = 60660, classes: int = 33, per_class: int = 90, queries: int = 3013,
+                      seed: int = 2026, device="cuda"):
+    """Class-centroid references and held-out queries: for each of `classes`
+    classes, `per_class` samples from the generator above are averaged into
+    one reference (binary32 mean); the queries are independent samples with
+    classes drawn uniformly. Returns (centroids classes x d, queries, query
+    classes)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    prof = class_profiles(d, classes, gen, device)
+    cents = torch.empty(classes, d, dtype=torch.float32, device=device)
+    for k in range(classes):
+        x = prof[k].expand(per_class, d) * torch.exp(0.7 * torch.randn(per_class, d, generator=gen, device=device))
+        x = x * (torch.rand(per_class, d, generator=gen, device=device) >= 0.1)
+        cents[k] = torch.log2(x + 1.0).mean(0)
+    labels = torch.randint(classes, (queries,), generator=gen, device=device)
+    q = torch.empty(queries, d, dtype=torch.float32, device=device)
+    for s0 in range(0, queries, 1024):
+        e = min(queries, s0 + 1024)
+        x = prof[labels[s0:e]] * torch.exp(0.7 * torch.randn(e - s0, d, generator=gen, device=device))
+        x = x * (torch.rand(e - s0, d, generator=gen, device=device) >= 0.1)
+        q[s0:e] = torch.log2(x + 1.0)
+    return cents, q, labels
