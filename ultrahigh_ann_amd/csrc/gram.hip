@@ -42,6 +42,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBK = 64;
 constexpr int kPad = 8;                 // bf16 elements of row padding in LDS
@@ -142,17 +143,17 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nblocks)
 
 // Global -> register staging of one K step: an operand tile is ROWS x 64
 // bf16 = ROWS * 8 chunks of 16 bytes; thread tid moves chunks tid + NT*c.
-#define UHANN_GRAM_LOAD(k0)                                                         \
-    _Pragma("unroll") for (int c = 0; c < CA; ++c) ra[c] =                          \
-        *reinterpret_cast<const uint4*>(qbase + c * qstep + (k0));                  \
-    _Pragma("unroll") for (int c = 0; c < CB; ++c) rb[c] =                          \
-        *reinterpret_cast<const uint4*>(vbase + c * vstep + (k0));                  \
+#define UHANN_GRAM_LOAD(k0, RA, RB)                                                 \
+    _Pragma("unroll") for (int c = 0; c < CA; ++c) RA[c] =                          \
+        *reinterpret_cast<const u32x4*>(qbase + c * qstep + (k0));                  \
+    _Pragma("unroll") for (int c = 0; c < CB; ++c) RB[c] =                          \
+        *reinterpret_cast<const u32x4*>(vbase + c * vstep + (k0));                  \
     __builtin_amdgcn_sched_barrier(0);
-#define UHANN_GRAM_STORE(buf)                                                       \
+#define UHANN_GRAM_STORE(buf, RA, RB)                                               \
     _Pragma("unroll") for (int c = 0; c < CA; ++c)                                  \
-        *reinterpret_cast<uint4*>(As + (buf) * BM * kLdsRow + lds0 + c * kLdsStep) = ra[c]; \
+        *reinterpret_cast<u32x4*>(As + (buf) * BM * kLdsRow + lds0 + c * kLdsStep) = RA[c]; \
     _Pragma("unroll") for (int c = 0; c < CB; ++c)                                  \
-        *reinterpret_cast<uint4*>(Bs + (buf) * BN * kLdsRow + lds0 + c * kLdsStep) = rb[c];
+        *reinterpret_cast<u32x4*>(Bs + (buf) * BN * kLdsRow + lds0 + c * kLdsStep) = RB[c];
 // Four K=16 MFMA steps over one LDS stage for the wave's TM x TN 32x32 tiles:
 // TM + TN fragment reads feed TM * TN MFMAs.
 #define UHANN_GRAM_COMPUTE(buf)                                                     \
@@ -209,7 +210,7 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
     const int wm = (wave / WN) * 32 * TM;  // wave's offsets inside the block tile
     const int wn = (wave % WN) * 32 * TN;
 
-    uint4 ra[CA], rb[CB];
+    u32x4 ra[CA], rb[CB];
     // Chunk c of this thread is row tid/8 + c*NT/8, 16-byte column chunk
     // tid % 8, of the block's tile: one base address per operand plus
     // compile-time row strides. Q and V hold whole tiles (rows padded).
@@ -232,14 +233,14 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
 
-    UHANN_GRAM_LOAD(0)
-    UHANN_GRAM_STORE(0)
+    UHANN_GRAM_LOAD(0, ra, rb)
+    UHANN_GRAM_STORE(0, ra, rb)
     __syncthreads();
     int cur = 0;
     for (uint32_t k0 = kBK; k0 < K; k0 += kBK) {
-        UHANN_GRAM_LOAD(k0)
+        UHANN_GRAM_LOAD(k0, ra, rb)
         UHANN_GRAM_COMPUTE(cur)
-        UHANN_GRAM_STORE(cur ^ 1)
+        UHANN_GRAM_STORE(cur ^ 1, ra, rb)
         __syncthreads();
         cur ^= 1;
     }
@@ -650,7 +651,7 @@ gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const u
     const int wm = (wave / WN) * 64;
     const int wn = (wave % WN) * 64;
 
-    uint4 ra[CA], rb[CB];
+    u32x4 ra[CA], rb[CB];
     const uint8_t* qbase = Q + static_cast<uint64_t>(m0 + tid / 8) * ldq + (tid % 8) * 16;
     const uint8_t* vbase = V + static_cast<uint64_t>(n0 + tid / 8) * ldv + (tid % 8) * 16;
     const uint64_t qstep = static_cast<uint64_t>(NT / 8) * ldq;
@@ -669,30 +670,21 @@ gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const u
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
 
     auto frag = [](const uint8_t* p) {
-        const uint4 lo = *reinterpret_cast<const uint4*>(p);
-        const uint4 hi = *reinterpret_cast<const uint4*>(p + 16);
-        intx8 f;
-        f[0] = static_cast<int>(lo.x);
-        f[1] = static_cast<int>(lo.y);
-        f[2] = static_cast<int>(lo.z);
-        f[3] = static_cast<int>(lo.w);
-        f[4] = static_cast<int>(hi.x);
-        f[5] = static_cast<int>(hi.y);
-        f[6] = static_cast<int>(hi.z);
-        f[7] = static_cast<int>(hi.w);
-        return f;
+        const u32x4 lo = *reinterpret_cast<const u32x4*>(p);
+        const u32x4 hi = *reinterpret_cast<const u32x4*>(p + 16);
+        return __builtin_bit_cast(intx8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
     };
 #define UHANN_FP8_LOAD(k0)                                                                    \
     _Pragma("unroll") for (int c = 0; c < CA; ++c) ra[c] =                                    \
-        *reinterpret_cast<const uint4*>(qbase + c * qstep + (k0));                            \
+        *reinterpret_cast<const u32x4*>(qbase + c * qstep + (k0));                            \
     _Pragma("unroll") for (int c = 0; c < CB; ++c) rb[c] =                                    \
-        *reinterpret_cast<const uint4*>(vbase + c * vstep + (k0));                            \
+        *reinterpret_cast<const u32x4*>(vbase + c * vstep + (k0));                            \
     __builtin_amdgcn_sched_barrier(0);
 #define UHANN_FP8_STORE(buf)                                                                  \
     _Pragma("unroll") for (int c = 0; c < CA; ++c)                                            \
-        *reinterpret_cast<uint4*>(As + (buf) * BM * kLdsRow8 + lds0 + c * kLdsStep) = ra[c];  \
+        *reinterpret_cast<u32x4*>(As + (buf) * BM * kLdsRow8 + lds0 + c * kLdsStep) = ra[c];  \
     _Pragma("unroll") for (int c = 0; c < CB; ++c)                                            \
-        *reinterpret_cast<uint4*>(Bs + (buf) * BN * kLdsRow8 + lds0 + c * kLdsStep) = rb[c];
+        *reinterpret_cast<u32x4*>(Bs + (buf) * BN * kLdsRow8 + lds0 + c * kLdsStep) = rb[c];
 #define UHANN_FP8_COMPUTE(buf)                                                                \
     _Pragma("unroll") for (int kk = 0; kk < kBK8; kk += 64)                                   \
     {                                                                                         \
