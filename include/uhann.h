@@ -309,8 +309,11 @@ int uhann_importance_probabilities_device(int metric, const float* host_refs, ui
 /* The mixed strategy's Gram estimates ||q||^2 + ||v||^2 - 2 <bf16(q), bf16(v)>
  * (binary32 MFMA accumulation) for count device queries against every
  * reference, written to device_out (count rows of ld_out >= rows floats) on
- * `stream`. tile selects the Gram block shape (0..7, -1 = the default).
- * Needs uhann_scan_enable_mixed. No counterpart in the reference (its GEMM
+ * `stream`. tile -1 runs what the queries run: the default block shape,
+ * with the tiles of a partial last round split along K over the idle
+ * compute units (their parts summed in order afterwards); 0..7 force a
+ * block shape and keep every tile whole (one sequential K accumulation per
+ * output, the same bits for every shape). Needs uhann_scan_enable_mixed. No counterpart in the reference (its GEMM
  * strategy keeps the unrefined SGEMM distances internal): for diagnostics
  * such as full distance tables and for the tests of each block shape. */
 int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,

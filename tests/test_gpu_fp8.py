@@ -59,7 +59,8 @@ def _case(seed, r, d, b):
     return refs, q
 
 
-@pytest.mark.parametrize("r,d,b", [(300, 1000, 40), (77, 3000, 9), (40, 130, 70)])
+# (300, 4096, 40) and (600, 8192, 300) split their tiles along K (fewer tiles than compute units)
+@pytest.mark.parametrize("r,d,b", [(300, 1000, 40), (77, 3000, 9), (40, 130, 70), (300, 4096, 40), (600, 8192, 300)])
 def test_fp8_estimates_match_emulation(uh, r, d, b):
     import torch
 
@@ -92,14 +93,16 @@ def test_fp8_estimates_match_emulation(uh, r, d, b):
     assert np.any(np.abs(est - wrong) > bound)
 
 
-def test_fp8_estimates_exact_on_small_integers(uh):
+@pytest.mark.parametrize("d", [384, 4096])
+def test_fp8_estimates_exact_on_small_integers(uh, d):
     """Integers |x| <= 8 are exact in e4m3 under the power-of-two scale and
-    every partial sum is representable: the estimates are exact."""
+    every partial sum is representable: the estimates are exact (d = 4096
+    also sums K parts)."""
     import torch
 
     g = rng(1)
-    refs = g.integers(-8, 9, (40, 384)).astype(F32)
-    q = g.integers(-8, 9, (33, 384)).astype(F32)
+    refs = g.integers(-8, 9, (40, d)).astype(F32)
+    q = g.integers(-8, 9, (33, d)).astype(F32)
     index = uh.HipExactL2Index(refs)
     index.enable_mixed(uh.STRATEGY_GRAM_FP8)
     ws = index.make_query_workspace(33)

@@ -161,3 +161,47 @@ def test_gram_estimates_every_tile(uh, r, d, b):
     trunc = lambda a: (np.ascontiguousarray(a, F32).view(np.uint32) & 0xFFFF0000).view(F32).astype(np.float64)
     wrong = nq[:, None] + nv[None, :] - 2.0 * (trunc(q) @ trunc(refs).T)
     assert np.any(np.abs(est - wrong) > bound)
+
+
+@pytest.mark.parametrize("r,d,b", [(300, 4096, 70), (4200, 4096, 4100), (700, 2048, 600)])
+def test_gram_split_tail(uh, r, d, b):
+    """The queries' Gram configuration splits the tiles of a partial last
+    round along K (here: every tile when there are fewer tiles than compute
+    units, and the 33-tile tail of a 289-tile grid) and sums the parts in
+    order. Tiles that stay whole keep the whole-tile bits; split tiles stay
+    within the binary32 accumulation bound of the exact
+    ||q||^2 + ||v||^2 - 2 <bf16(q), bf16(v)>, and differ from the whole-tile
+    bits somewhere (the split ran)."""
+    import torch
+
+    g = rng(r + d + b)
+    refs = (g.random((r, d), dtype=np.float32) * 4 - 1).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 4 - 1).astype(F32)
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed()
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    outs = {}
+    for tile in (3, -1, -1):
+        out = torch.full((b, r + 1), float("nan"), dtype=torch.float32, device="cuda")
+        index.gram_estimates(dq, b, ws, out, tile)
+        torch.cuda.synchronize()
+        outs.setdefault(tile, []).append(out.cpu().numpy())
+    whole = outs[3][0]
+    split, split2 = outs[-1]
+    assert np.all(np.isnan(split[:, r:]))
+    np.testing.assert_array_equal(split.view(np.uint32), split2.view(np.uint32))  # deterministic
+    split = split[:, :r].astype(np.float64)
+    assert not np.array_equal(split, whole[:, :r].astype(np.float64))
+    qb, vb = bf16_round(q), bf16_round(refs)
+    nq = (q.astype(np.float64) ** 2).sum(1)
+    nv = (refs.astype(np.float64) ** 2).sum(1)
+    want = nq[:, None] + nv[None, :] - 2.0 * (qb @ vb.T)
+    steps = (d + 63) // 64 * 4
+    bound = 2.0 * (steps + 8) * 2.0**-24 * (np.abs(qb) @ np.abs(vb).T) + 4 * 2.0**-24 * (nq[:, None] + nv[None, :])
+    assert np.all(np.abs(split - want) <= bound)
+    if b * r > 256 * 256 * 256:
+        # 289 tiles: the first 256 in the grouped order stay whole, so most
+        # estimates keep the whole-tile bits
+        same = split == whole[:, :r].astype(np.float64)
+        assert same.mean() > 0.8

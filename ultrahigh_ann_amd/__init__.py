@@ -307,7 +307,10 @@ class _DenseScan:
     def gram_estimates(self, device_queries, count: int, workspace: QueryWorkspace, out, tile: int = -1,
                        stream=None, ld_queries=None):
         """Gram estimates ||q||^2 + ||v||^2 - 2 <bf16(q), bf16(v)> into the
-        device tensor out (count x rows, row stride out.stride(0))."""
+        device tensor out (count x rows, row stride out.stride(0)). tile -1
+        runs the queries' configuration (default block shape, the tiles of a
+        partial last round split along K); 0..7 force a block shape with every
+        tile accumulated whole."""
         ld = ld_queries if ld_queries is not None else self.dim
         check(
             _native.lib().uhann_scan_gram_estimates(

@@ -399,6 +399,26 @@ void Scan::enable_mixed(int strategy)
     gram = std::move(g);
 }
 
+namespace {
+
+uint32_t device_cus(int device)
+{
+    int cus = 0;
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
+              "cannot read the compute unit count");
+    return cus > 0 ? static_cast<uint32_t>(cus) : 0U;
+}
+
+}  // namespace
+
+void Workspace::reserve_partials(uint64_t floats)
+{
+    const uint64_t have = gram_partials.bytes() / sizeof(float);
+    if (floats <= have) return;
+    gram_partials = DeviceBuffer<float>(device, floats);
+    payload_bytes += (floats - have) * sizeof(float);
+}
+
 void Workspace::reserve_mixed(const Scan& scan)
 {
     if (scan.gram == nullptr || q16.bytes() != 0) return;
@@ -407,6 +427,9 @@ void Workspace::reserve_mixed(const Scan& scan)
     q16 = DeviceBuffer<__bf16>(device, checked_mul(mrows, scan.gram->ldk, "bf16 query workspace"));
     qstats = DeviceBuffer<kern::RowStats>(device, mrows);
     payload_bytes += q16.bytes() + qstats.bytes();
+    reserve_partials(kern::gram_split_floats(static_cast<uint32_t>(max_batch), static_cast<uint32_t>(scan.rows),
+                                             static_cast<uint32_t>(scan.gram->ldk / kern::gram_k_padding()),
+                                             device_cus(device)));
     reserve_common(scan);
 }
 
@@ -418,6 +441,9 @@ void Workspace::reserve_fp8(const Scan& scan)
     q8 = DeviceBuffer<uint8_t>(device, checked_mul(mrows, scan.fp8->ldk8, "fp8 query workspace"));
     q8stats = DeviceBuffer<kern::RowStats>(device, mrows);
     payload_bytes += q8.bytes() + q8stats.bytes();
+    reserve_partials(kern::gram_split_floats(static_cast<uint32_t>(max_batch), static_cast<uint32_t>(scan.rows),
+                                             static_cast<uint32_t>(scan.fp8->ldk8 / kern::gram_fp8_k_padding()),
+                                             device_cus(device)));
     reserve_common(scan);
 }
 
@@ -453,7 +479,8 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
     ws.mark(1, stream);
     kern::gram_bf16(ws.q16.data(), g.ldk, n, g.refs16.data(), g.ldk, static_cast<uint32_t>(scan.rows),
                     static_cast<uint32_t>(g.ldk), ws.qstats.data(), g.stats.data(), ws.approx.data(),
-                    scan.rows, gram_tile(), stream);
+                    scan.rows, gram_tile(), ws.gram_partials.data(), ws.gram_partials.bytes() / sizeof(float),
+                    stream);
     ws.mark(2, stream);
     kern::zero_u32(ws.counters.data(), 8, stream);
     kern::SelectArgs a;
@@ -521,7 +548,7 @@ void fp8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t 
     ws.mark(1, stream);
     kern::gram_fp8(ws.q8.data(), f.ldk8, n, f.refs8.data(), f.ldk8, static_cast<uint32_t>(scan.rows),
                    static_cast<uint32_t>(f.ldk8), ws.q8stats.data(), f.stats.data(), ws.approx.data(), scan.rows,
-                   stream);
+                   ws.gram_partials.data(), ws.gram_partials.bytes() / sizeof(float), stream);
     ws.mark(2, stream);
     kern::zero_u32(ws.counters.data(), 8, stream);
     kern::SelectArgs a;
@@ -785,7 +812,8 @@ int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const
                            ws->qstats.data(), s);
         kern::gram_bf16(ws->q16.data(), g.ldk, n, g.refs16.data(), g.ldk, static_cast<uint32_t>(scan->rows),
                         static_cast<uint32_t>(g.ldk), ws->qstats.data(), g.stats.data(), device_out, ld_out,
-                        tile < 0 ? gram_tile() : tile, s);
+                        tile < 0 ? gram_tile() : tile, tile < 0 ? ws->gram_partials.data() : nullptr,
+                        ws->gram_partials.bytes() / sizeof(float), s);
     });
 }
 
@@ -810,7 +838,8 @@ int uhann_scan_fp8_estimates(const uhann_scan* scan, uhann_workspace* ws, const 
         kern::to_fp8_rows(device_queries, ld_queries, n, static_cast<uint32_t>(scan->dim), ws->q8.data(), f.ldk8,
                           ws->q8stats.data(), s);
         kern::gram_fp8(ws->q8.data(), f.ldk8, n, f.refs8.data(), f.ldk8, static_cast<uint32_t>(scan->rows),
-                       static_cast<uint32_t>(f.ldk8), ws->q8stats.data(), f.stats.data(), device_out, ld_out, s);
+                       static_cast<uint32_t>(f.ldk8), ws->q8stats.data(), f.stats.data(), device_out, ld_out,
+                       ws->gram_partials.data(), ws->gram_partials.bytes() / sizeof(float), s);
     });
 }
 

@@ -103,6 +103,8 @@ struct Workspace {
     DeviceBuffer<float> esc_dist;
     DeviceBuffer<uint8_t> q8;           // FP8 strategy: e4m3 queries
     DeviceBuffer<kern::RowStats> q8stats;
+    DeviceBuffer<float> gram_partials;  // K-split partial sums of the Gram tail tiles
+    void reserve_partials(uint64_t floats);
     void reserve_mixed(const Scan& scan);
     void reserve_fp8(const Scan& scan);
     void reserve_common(const Scan& scan);

@@ -31,6 +31,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -141,6 +142,136 @@ __device__ __forceinline__ uint32_t xcd_logical(uint32_t b, uint32_t nblocks)
     return x * per + (x < rem ? x : rem) + b / 8U;
 }
 
+// The work of one Gram block: logical tile `tile` (grouped order, see
+// tile_origin), K range [k_begin, k_end), and for a split tile its K part
+// (-1 for a whole tile). Blocks [0, full) take whole tiles; blocks past that
+// take K parts of the tail tiles, part-major so that blocks running together
+// share the same K range of neighbouring tiles.
+struct GramBlock {
+    uint32_t tile;
+    uint32_t k_begin;
+    uint32_t k_end;
+    int part;
+};
+
+__device__ __forceinline__ GramBlock gram_block(uint32_t b, const GramSplit& sp, uint32_t K, uint32_t kstep)
+{
+    if (b < sp.full) return {xcd_logical(b, sp.full), 0U, K, -1};
+    const uint32_t u = xcd_logical(b - sp.full, sp.tail * sp.parts);
+    const uint32_t part = u / sp.tail;
+    const uint32_t kb = part * sp.part_steps * kstep;
+    return {sp.full + u % sp.tail, kb, min(K, kb + sp.part_steps * kstep), static_cast<int>(part)};
+}
+
+// Output origin (in tiles) of logical tile L: groups of group_m tile rows are
+// swept column by column, so consecutive logical tiles share operand tiles.
+__device__ __forceinline__ uint2 tile_origin(uint32_t L, uint32_t tiles_m, uint32_t tiles_n, uint32_t group_m)
+{
+    const uint32_t span = group_m * tiles_n;
+    const uint32_t gid = L / span;
+    const uint32_t first_m = gid * group_m;
+    const uint32_t gsize = min(group_m, tiles_m - first_m);
+    const uint32_t within = L - gid * span;
+    return make_uint2(first_m + within % gsize, within / gsize);
+}
+
+// A split tile's partial accumulators: unit (tail tile, part) holds NT
+// threads x TM*TN*4 float4 chunks, chunk-major so that stores and loads are
+// coalesced. Chunk c = (i*TN + j)*4 + v/4 holds acc[i][j][v .. v+3].
+template <int NT, int TM, int TN>
+__device__ __forceinline__ void store_partials(float* __restrict__ dst, const floatx16 (&acc)[TM][TN], int tid)
+{
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int chunk = (i * TN + j) * 4 + c;
+                *reinterpret_cast<float4*>(dst + (static_cast<uint64_t>(chunk) * NT + tid) * 4) =
+                    make_float4(acc[i][j][4 * c], acc[i][j][4 * c + 1], acc[i][j][4 * c + 2], acc[i][j][4 * c + 3]);
+            }
+}
+
+// out[m][n] = ||q||^2 + ||v||^2 - 2 acc (SCALED: - 2 (s_q s_v acc)) for the
+// wave's TM x TN 32x32 accumulators at rows mw.., columns nw... D layout of
+// the 32x32 MFMA: lane holds column (lane % 32) and rows
+// 8*(v/4) + 4*(lane/32) + v%4 for its 16 values v.
+template <int TM, int TN, bool SCALED>
+__device__ __forceinline__ void gram_epilogue(const floatx16 (&acc)[TM][TN], uint32_t mw, uint32_t nw, int lane,
+                                              uint32_t M, uint32_t N, const RowStats* __restrict__ qs,
+                                              const RowStats* __restrict__ vs, float* __restrict__ out,
+                                              uint64_t ldo)
+{
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const uint32_t n = nw + j * 32 + (lane % 32);
+            if (n >= N) continue;
+            const float nv = vs[n].norm2;
+            const float sv = SCALED ? vs[n].scale : 1.0F;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t m = mw + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (m < M) {
+                    const float base = qs[m].norm2 + nv;
+                    float r;
+                    if constexpr (SCALED) {
+                        r = base - 2.0F * ((qs[m].scale * sv) * acc[i][j][v]);
+                    } else {
+                        r = base - 2.0F * acc[i][j][v];
+                    }
+                    out[static_cast<uint64_t>(m) * ldo + n] = r;
+                }
+            }
+        }
+    }
+}
+
+// Sums the K parts of one split tile in part order and applies the epilogue
+// (one block per tail tile, the same thread layout as the Gram kernel).
+template <int WM, int WN, int TM, int TN, bool SCALED>
+__global__ void __launch_bounds__(64 * WM * WN)
+gram_fixup_kernel(GramSplit sp, uint32_t M, uint32_t N, const RowStats* __restrict__ qs,
+                  const RowStats* __restrict__ vs, float* __restrict__ out, uint64_t ldo, uint32_t tiles_m,
+                  uint32_t tiles_n, uint32_t group_m)
+{
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BM = 32 * TM * WM;
+    constexpr int BN = 32 * TN * WN;
+    const int tid = threadIdx.x;
+    const int lane = tid % 64;
+    const int wave = tid / 64;
+    const uint2 o = tile_origin(sp.full + blockIdx.x, tiles_m, tiles_n, group_m);
+    floatx16 acc[TM][TN];
+    for (uint32_t p = 0; p < sp.parts; ++p) {
+        const float* src = sp.partials + (static_cast<uint64_t>(blockIdx.x) * sp.parts + p) * (BM * BN);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int chunk = (i * TN + j) * 4 + c;
+                    const float4 x = *reinterpret_cast<const float4*>(src + (static_cast<uint64_t>(chunk) * NT + tid) * 4);
+                    if (p == 0) {
+                        acc[i][j][4 * c] = x.x;
+                        acc[i][j][4 * c + 1] = x.y;
+                        acc[i][j][4 * c + 2] = x.z;
+                        acc[i][j][4 * c + 3] = x.w;
+                    } else {
+                        acc[i][j][4 * c] += x.x;
+                        acc[i][j][4 * c + 1] += x.y;
+                        acc[i][j][4 * c + 2] += x.z;
+                        acc[i][j][4 * c + 3] += x.w;
+                    }
+                }
+    }
+    gram_epilogue<TM, TN, SCALED>(acc, o.x * BM + (wave / WN) * 32 * TM, o.y * BN + (wave % WN) * 32 * TN, lane, M,
+                                  N, qs, vs, out, ldo);
+}
+
 // Global -> register staging of one K step: an operand tile is ROWS x 64
 // bf16 = ROWS * 8 chunks of 16 bytes; thread tid moves chunks tid + NT*c.
 #define UHANN_GRAM_LOAD(k0, RA, RB)                                                 \
@@ -184,7 +315,7 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
                  const __bf16* __restrict__ V, uint64_t ldv, uint32_t N, uint32_t K,
                  const RowStats* __restrict__ qs, const RowStats* __restrict__ vs,
                  float* __restrict__ out, uint64_t ldo, uint32_t tiles_m, uint32_t tiles_n,
-                 uint32_t group_m)
+                 uint32_t group_m, GramSplit sp)
 {
     constexpr int NT = 64 * WM * WN;
     constexpr int BM = 32 * TM * WM;
@@ -195,14 +326,10 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
     __shared__ __attribute__((aligned(16))) __bf16 As[2 * BM * kLdsRow];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[2 * BN * kLdsRow];
 
-    const uint32_t L = xcd_logical(blockIdx.x, tiles_m * tiles_n);
-    const uint32_t span = group_m * tiles_n;
-    const uint32_t gid = L / span;
-    const uint32_t first_m = gid * group_m;
-    const uint32_t gsize = min(group_m, tiles_m - first_m);
-    const uint32_t within = L - gid * span;
-    const uint32_t m0 = (first_m + within % gsize) * BM;
-    const uint32_t n0 = (within / gsize) * BN;
+    const GramBlock gb = gram_block(blockIdx.x, sp, K, kBK);
+    const uint2 o = tile_origin(gb.tile, tiles_m, tiles_n, group_m);
+    const uint32_t m0 = o.x * BM;
+    const uint32_t n0 = o.y * BN;
 
     const int tid = threadIdx.x;
     const int lane = tid % 64;
@@ -233,11 +360,11 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
 
-    UHANN_GRAM_LOAD(0, ra, rb)
+    UHANN_GRAM_LOAD(gb.k_begin, ra, rb)
     UHANN_GRAM_STORE(0, ra, rb)
     __syncthreads();
     int cur = 0;
-    for (uint32_t k0 = kBK; k0 < K; k0 += kBK) {
+    for (uint32_t k0 = gb.k_begin + kBK; k0 < gb.k_end; k0 += kBK) {
         UHANN_GRAM_LOAD(k0, ra, rb)
         UHANN_GRAM_COMPUTE(cur)
         UHANN_GRAM_STORE(cur ^ 1, ra, rb)
@@ -246,25 +373,12 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
     }
     UHANN_GRAM_COMPUTE(cur)
 
-    // D layout of the 32x32 MFMA: lane holds column (lane % 32) and rows
-    // 8*(v/4) + 4*(lane/32) + v%4 for its 16 values v.
-#pragma unroll
-    for (int i = 0; i < TM; ++i) {
-#pragma unroll
-        for (int j = 0; j < TN; ++j) {
-            const uint32_t n = n0 + wn + j * 32 + (lane % 32);
-            if (n >= N) continue;
-            const float nv = vs[n].norm2;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const uint32_t m = m0 + wm + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-                if (m < M) {
-                    const float base = qs[m].norm2 + nv;
-                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * acc[i][j][v];
-                }
-            }
-        }
+    if (gb.part >= 0) {
+        const uint64_t unit = static_cast<uint64_t>(gb.tile - sp.full) * sp.parts + gb.part;
+        store_partials<NT, TM, TN>(sp.partials + unit * (BM * BN), acc, tid);
+        return;
     }
+    gram_epilogue<TM, TN, false>(acc, m0 + wm, n0 + wn, lane, M, N, qs, vs, out, ldo);
 }
 
 #undef UHANN_GRAM_LOAD
@@ -628,7 +742,7 @@ __global__ void __launch_bounds__(1024)
 gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const uint8_t* __restrict__ V,
                 uint64_t ldv, uint32_t N, uint32_t K, const RowStats* __restrict__ qs,
                 const RowStats* __restrict__ vs, float* __restrict__ out, uint64_t ldo, uint32_t tiles_m,
-                uint32_t tiles_n, uint32_t group_m)
+                uint32_t tiles_n, uint32_t group_m, GramSplit sp)
 {
     constexpr int WN = 4, NT = 1024, BM = 256, BN = 256;
     constexpr int CA = BM * 8 / NT;  // 16-byte chunks per thread per K step
@@ -636,14 +750,10 @@ gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const u
     __shared__ __attribute__((aligned(16))) uint8_t As[2 * BM * kLdsRow8];
     __shared__ __attribute__((aligned(16))) uint8_t Bs[2 * BN * kLdsRow8];
 
-    const uint32_t L = xcd_logical(blockIdx.x, tiles_m * tiles_n);
-    const uint32_t span = group_m * tiles_n;
-    const uint32_t gid = L / span;
-    const uint32_t first_m = gid * group_m;
-    const uint32_t gsize = min(group_m, tiles_m - first_m);
-    const uint32_t within = L - gid * span;
-    const uint32_t m0 = (first_m + within % gsize) * BM;
-    const uint32_t n0 = (within / gsize) * BN;
+    const GramBlock gb = gram_block(blockIdx.x, sp, K, kBK8);
+    const uint2 o = tile_origin(gb.tile, tiles_m, tiles_n, group_m);
+    const uint32_t m0 = o.x * BM;
+    const uint32_t n0 = o.y * BN;
 
     const int tid = threadIdx.x;
     const int lane = tid % 64;
@@ -700,11 +810,11 @@ gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const u
         acc[1][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, b1, acc[1][1], 0, 0, 0, 127, 0, 127); \
     }
 
-    UHANN_FP8_LOAD(0)
+    UHANN_FP8_LOAD(gb.k_begin)
     UHANN_FP8_STORE(0)
     __syncthreads();
     int cur = 0;
-    for (uint32_t k0 = kBK8; k0 < K; k0 += kBK8) {
+    for (uint32_t k0 = gb.k_begin + kBK8; k0 < gb.k_end; k0 += kBK8) {
         UHANN_FP8_LOAD(k0)
         UHANN_FP8_COMPUTE(cur)
         UHANN_FP8_STORE(cur ^ 1)
@@ -716,24 +826,12 @@ gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const u
 #undef UHANN_FP8_STORE
 #undef UHANN_FP8_COMPUTE
 
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const uint32_t n = n0 + wn + j * 32 + (lane % 32);
-            if (n >= N) continue;
-            const RowStats vn = vs[n];
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const uint32_t m = m0 + wm + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-                if (m < M) {
-                    const float base = qs[m].norm2 + vn.norm2;
-                    const float sc = qs[m].scale * vn.scale;
-                    out[static_cast<uint64_t>(m) * ldo + n] = base - 2.0F * (sc * acc[i][j][v]);
-                }
-            }
-        }
+    if (gb.part >= 0) {
+        const uint64_t unit = static_cast<uint64_t>(gb.tile - sp.full) * sp.parts + gb.part;
+        store_partials<NT, 2, 2>(sp.partials + unit * (BM * BN), acc, tid);
+        return;
     }
+    gram_epilogue<2, 2, true>(acc, m0 + wm, n0 + wn, lane, M, N, qs, vs, out, ldo);
 }
 
 // Top-kShortlist selection per query (64-bit keys (ordered estimate << 32 |
@@ -858,9 +956,81 @@ void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim
     check_launch("bf16 conversion kernel");
 }
 
+namespace {
+
+constexpr uint32_t kMaxParts = 16;
+constexpr uint32_t kMinPartSteps = 8;
+
+uint32_t current_device_cus()
+{
+    int dev = 0;
+    int cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return cus > 0 ? static_cast<uint32_t>(cus) : 0U;
+}
+
+// The plan for this launch if the scratch holds it, else every tile whole.
+GramSplit split_for(uint32_t tiles, uint32_t ksteps, uint64_t block_outputs, float* partials,
+                    uint64_t partial_floats)
+{
+    GramSplit none;
+    none.full = tiles;
+    if (partials == nullptr) return none;
+    GramSplit sp = gram_split_plan(tiles, ksteps, current_device_cus());
+    if (sp.tail == 0 || static_cast<uint64_t>(sp.tail) * sp.parts * block_outputs > partial_floats) return none;
+    sp.partials = partials;
+    return sp;
+}
+
+}  // namespace
+
+GramSplit gram_split_plan(uint32_t tiles, uint32_t ksteps, uint32_t cus)
+{
+    GramSplit sp;
+    sp.full = tiles;
+    if (cus == 0) return sp;
+    const uint32_t full = tiles / cus * cus;
+    const uint32_t tail = tiles - full;
+    if (tail == 0) return sp;
+    // Rounds the tail costs with p parts (in whole-tile times): ceil(tail p / cus) / p.
+    uint32_t best_p = 1;
+    uint64_t best_num = 1, best_den = 1;
+    const uint32_t max_p = std::min(kMaxParts, ksteps / kMinPartSteps);
+    for (uint32_t p = 2; p <= max_p; ++p) {
+        const uint64_t units = static_cast<uint64_t>(tail) * p;
+        if (units > 3ULL * cus) break;
+        const uint64_t rounds = (units + cus - 1) / cus;
+        if (rounds * best_den < best_num * p) {
+            best_num = rounds;
+            best_den = p;
+            best_p = p;
+        }
+    }
+    if (best_p == 1) return sp;
+    sp.full = full;
+    sp.tail = tail;
+    sp.part_steps = (ksteps + best_p - 1) / best_p;
+    sp.parts = (ksteps + sp.part_steps - 1) / sp.part_steps;
+    return sp;
+}
+
+uint64_t gram_split_floats(uint32_t max_m, uint32_t n, uint32_t k_steps, uint32_t cus)
+{
+    const uint32_t tiles_n = (n + 255U) / 256U;
+    uint64_t need = 0;
+    for (uint32_t tm = 1; tm <= (max_m + 255U) / 256U; ++tm) {
+        const GramSplit sp = gram_split_plan(tm * tiles_n, k_steps, cus);
+        need = std::max(need, static_cast<uint64_t>(sp.tail) * sp.parts * gram_block_outputs());
+    }
+    return need;
+}
+
+uint32_t gram_block_outputs() { return 256U * 256U; }
+
 void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint64_t ldv, uint32_t n,
                uint32_t k, const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo,
-               int tile, hipStream_t stream)
+               int tile, float* partials, uint64_t partial_floats, hipStream_t stream)
 {
     if (m == 0 || n == 0) return;
     if (k % kBK != 0) throw std::invalid_argument("Gram K must be a multiple of 64");
@@ -880,12 +1050,18 @@ void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint6
     const uint32_t tiles_m = (m + bm - 1) / bm;
     const uint32_t tiles_n = (n + bn - 1) / bn;
     const uint32_t group_m = 8;
-    const dim3 grid(tiles_m * tiles_n);
+    const GramSplit sp = split_for(tiles_m * tiles_n, k / kBK, static_cast<uint64_t>(bm) * bn, partials,
+                                   partial_floats);
+    const dim3 grid(sp.full + sp.tail * sp.parts);
     const dim3 block(64 * sh.wm * sh.wn);
 #define UHANN_GRAM_CASE(T, A, B, C, D)                                                                   \
     case T:                                                                                            \
         gram_bf16_kernel<A, B, C, D><<<grid, block, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo, \
-                                                                tiles_m, tiles_n, group_m);             \
+                                                                tiles_m, tiles_n, group_m, sp);         \
+        if (sp.tail > 0) {                                                                             \
+            gram_fixup_kernel<A, B, C, D, false><<<sp.tail, block, 0, stream>>>(sp, m, n, qs, vs, out, ldo, \
+                                                                               tiles_m, tiles_n, group_m); \
+        }                                                                                              \
         break;
     switch (tile) {
         UHANN_GRAM_CASE(0, 2, 2, 2, 2)
@@ -919,15 +1095,21 @@ void to_fp8_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim,
 }
 
 void gram_fp8(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uint64_t ldv, uint32_t n, uint32_t k,
-              const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo, hipStream_t stream)
+              const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo, float* partials,
+              uint64_t partial_floats, hipStream_t stream)
 {
     if (m == 0 || n == 0) return;
     if (k % kBK8 != 0) throw std::invalid_argument("FP8 Gram K must be a multiple of 128");
     if (ldq % 16 != 0 || ldv % 16 != 0) throw std::invalid_argument("FP8 Gram rows must be 16-byte aligned");
     const uint32_t tiles_m = (m + 255U) / 256U;
     const uint32_t tiles_n = (n + 255U) / 256U;
-    gram_fp8_kernel<<<tiles_m * tiles_n, 1024, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo, tiles_m,
-                                                           tiles_n, 8U);
+    const GramSplit sp = split_for(tiles_m * tiles_n, k / kBK8, gram_block_outputs(), partials, partial_floats);
+    gram_fp8_kernel<<<sp.full + sp.tail * sp.parts, 1024, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,
+                                                                      tiles_m, tiles_n, 8U, sp);
+    if (sp.tail > 0) {
+        gram_fixup_kernel<4, 4, 2, 2, true><<<sp.tail, 1024, 0, stream>>>(sp, m, n, qs, vs, out, ldo, tiles_m,
+                                                                          tiles_n, 8U);
+    }
     check_launch("fp8 Gram kernel");
 }
 

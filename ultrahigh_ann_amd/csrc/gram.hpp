@@ -28,6 +28,20 @@ struct MixedStats {
     unsigned int certified;      // FP8 shortlist: queries whose answer the error bound proves
 };
 
+// Split of a Gram grid's last, partial round of tiles along K. A grid of
+// `tiles` output tiles on C compute units (one block each) runs in
+// ceil(tiles / C) rounds; the tail tiles = tiles mod C of the last round
+// instead run as `parts` K ranges each, so the units fill the machine, and a
+// fix-up kernel sums each tile's parts in part order. Whole tiles keep the
+// sequential K accumulation; split tiles add their parts' sums.
+struct GramSplit {
+    uint32_t full = 0;          // whole tiles: the first `full` tiles of the grouped order
+    uint32_t tail = 0;          // tiles split along K
+    uint32_t parts = 1;         // K parts per tail tile
+    uint32_t part_steps = 0;    // K steps per part (the last part may have fewer)
+    float* partials = nullptr;  // tail * parts block tiles of binary32 partial sums
+};
+
 constexpr int kMaxCandidates = 64;
 constexpr int kShortlist = 16;  // FP8 strategy: candidates re-ranked per query
 
@@ -59,9 +73,11 @@ void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim
 // ceil(m / gram_m_tile()) * gram_m_tile() readable rows and v likewise for n:
 // the kernel reads whole tiles, rows past m / n only feed outputs that are
 // not written.
+// partials (or null: every tile whole) is scratch for the tail split of
+// up to partial_floats floats; a split that needs more is not used.
 void gram_bf16(const __bf16* q, uint64_t ldq, uint32_t m, const __bf16* v, uint64_t ldv, uint32_t n,
                uint32_t k, const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo,
-               int tile, hipStream_t stream);
+               int tile, float* partials, uint64_t partial_floats, hipStream_t stream);
 void select_rerank(const SelectArgs& args, hipStream_t stream);
 
 // binary32 rows -> e4m3 bytes (round to nearest even) with a power-of-two
@@ -73,7 +89,15 @@ void to_fp8_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim,
 // on v_mfma_scale_f32_32x32x64_f8f6f4 (binary32 accumulation); k a multiple
 // of gram_fp8_k_padding(), q and v padded to whole tiles like gram_bf16.
 void gram_fp8(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uint64_t ldv, uint32_t n, uint32_t k,
-              const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo, hipStream_t stream);
+              const RowStats* qs, const RowStats* vs, float* out, uint64_t ldo, float* partials,
+              uint64_t partial_floats, hipStream_t stream);
+// The K-split plan for `tiles` block tiles of ksteps K steps on `cus`
+// compute units (full == tiles: no split).
+GramSplit gram_split_plan(uint32_t tiles, uint32_t ksteps, uint32_t cus);
+// Scratch floats that the split of any batch of up to max_m queries against
+// n references needs with the 256 x 256 block tiles (k_steps K steps).
+uint64_t gram_split_floats(uint32_t max_m, uint32_t n, uint32_t k_steps, uint32_t cus);
+uint32_t gram_block_outputs();  // outputs per 256 x 256 block tile
 // Per query: the kShortlist rows with the smallest estimates (ties to the
 // lower row), re-ranked exactly in the binary32 reference order, binary64
 // for near ties; stats->certified counts queries where every other row's
