@@ -284,6 +284,19 @@ int uhann_build_coordinate_sample(uhann_rng* rng, const double* probabilities, u
                                   uint64_t repetitions, uhann_sampled_column* out, uint64_t capacity,
                                   uint64_t* count);
 
+/* compute_sampling_mass (coordinate_sampling/uniform_probabilities.cpp:
+ * 12-26): the dim probabilities summed in order. Invalid probability ->
+ * DOMAIN. */
+int uhann_sampling_mass(const double* probabilities, uint64_t dim, double* mass);
+
+/* make_mass_matched_uniform_probabilities (uniform_probabilities.cpp:28-45):
+ * dim copies of mass / dim into out, the probabilities of the uniform index
+ * kind (UniformL{1,2}AnnIndex, uniform_l{1,2}_ann_index.cpp; on the GPU the
+ * reference builds a CudaSampledCoordinateL{1,2}AnnIndex from their sample,
+ * nearest_neighbor_benchmark.cpp:755-768). Mass outside [0, dim] or not
+ * finite -> DOMAIN. */
+int uhann_uniform_probabilities(uint64_t dim, double mass, double* out);
+
 /* Projection matrix drawn after the sample: L2 JL (Rademacher sums,
  * l2_hierarchical_projection.cpp:53-77), L1 Cauchy
  * (l1_hierarchical_projection.cpp:54-77); out is projection_dim x count

@@ -54,6 +54,34 @@ inline CoordinateSample build_coordinate_sample(std::span<const double> probabil
     return sample;
 }
 
+// compute_sampling_mass (coordinate_sampling/uniform_probabilities.cpp:12-26):
+// the probabilities summed in order; a non-finite value or one outside
+// [0, 1] -> domain_error.
+inline double sampling_mass(std::span<const double> probabilities)
+{
+    double mass = 0.0;
+    for (const double p : probabilities) {
+        if (!std::isfinite(p)) throw std::domain_error("sampling probability must be finite");
+        if (p < 0.0 || p > 1.0) throw std::domain_error("invalid sampling probability");
+        mass += p;
+    }
+    return mass;
+}
+
+// make_mass_matched_uniform_probabilities (uniform_probabilities.cpp:28-45):
+// `dimension` copies of mass / dimension, the uniform sampler with the same
+// expected number of kept columns. A non-finite mass, or one outside
+// [0, dimension] -> domain_error.
+inline std::vector<double> uniform_probabilities(std::size_t dimension, double mass)
+{
+    if (!std::isfinite(mass)) throw std::domain_error("sampling mass must be finite");
+    if (mass < 0.0 || mass > static_cast<double>(dimension)) {
+        throw std::domain_error("sampling mass must be between zero and the dimension");
+    }
+    if (dimension == 0) return {};
+    return std::vector<double>(dimension, mass / static_cast<double>(dimension));
+}
+
 // Sample checks shared by the embeddings and the hierarchies
 // (l2_coordinate_embedding.cpp:41-60, l2_hierarchical_projection.cpp:29-51).
 inline std::size_t validate_sample(const CoordinateSample& sample, std::size_t dim)

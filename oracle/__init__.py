@@ -105,6 +105,10 @@ def ref():
             "ref_hierarchical_fp32_build": ([c_int, _f32p, c_uint64, c_uint64, _f64p, c_uint64, c_uint64,
                                              c_uint64, _u64p, _u64p, _f64p, _f32p, _f32p], c_int64),
             "ref_importance_probabilities": ([c_int, _f32p, c_uint64, c_uint64, _f64p], c_int),
+            "ref_sampling_mass": ([_f64p, c_uint64, _f64p], c_int),
+            "ref_uniform_probabilities": ([c_uint64, ctypes.c_double, _f64p], c_int),
+            "ref_uniform": ([c_int, _f32p, c_uint64, c_uint64, ctypes.c_double, c_uint64, c_uint64, _f32p,
+                             c_uint64, _u64p, _u64p], c_int),
         }
         for name, (args, r) in sig.items():
             f = getattr(L, name)

@@ -9,6 +9,9 @@
 // API and two of its build-time helpers (detail::build_*_hierarchical_projection*).
 
 #include "ultrahigh_ann/coordinate_sampling/coordinate_sampling.hpp"
+#include "ultrahigh_ann/coordinate_sampling/uniform_l1_ann_index.hpp"
+#include "ultrahigh_ann/coordinate_sampling/uniform_l2_ann_index.hpp"
+#include "ultrahigh_ann/coordinate_sampling/uniform_probabilities.hpp"
 #include "ultrahigh_ann/core/dense_l2_scan_index.hpp"
 #include "ultrahigh_ann/core/dense_matrix.hpp"
 #include "ultrahigh_ann/exact/l1/exact_l1_index.hpp"
@@ -119,6 +122,42 @@ int ref_flat(int metric_l1, const float* refs, uint64_t r, uint64_t d,
             std::vector<std::size_t> res(b);
             index.query_batch(std::span<const float>(queries, b * d), b, res);
             for (uint64_t i = 0; i < b; ++i) out[i] = res[i];
+        }
+    });
+}
+
+// compute_sampling_mass; 1 on an exception (domain_error for bad values).
+int ref_sampling_mass(const double* probs, uint64_t d, double* mass)
+{
+    return guarded([&] { *mass = uh::compute_sampling_mass(std::span<const double>(probs, d)); });
+}
+
+// make_mass_matched_uniform_probabilities into out (d doubles).
+int ref_uniform_probabilities(uint64_t d, double mass, double* out)
+{
+    return guarded([&] {
+        const auto p = uh::make_mass_matched_uniform_probabilities(d, mass);
+        for (std::size_t i = 0; i < p.size(); ++i) out[i] = p[i];
+    });
+}
+
+// UniformL{1,2}AnnIndex(input, mass, t, mt19937_64(seed)).query per query;
+// usage[0..1] = unique query coordinates, sampled multiplicity.
+int ref_uniform(int metric_l1, const float* refs, uint64_t r, uint64_t d, double mass, uint64_t t,
+                uint64_t seed, const float* queries, uint64_t b, uint64_t* out, uint64_t* usage)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        std::mt19937_64 eng(seed);
+        auto run = [&](const auto& index) {
+            for (uint64_t i = 0; i < b; ++i) out[i] = index.query(std::span<const float>(queries + i * d, d));
+            usage[0] = index.space_usage().unique_query_coordinates;
+            usage[1] = index.space_usage().sampled_multiplicity;
+        };
+        if (metric_l1) {
+            run(uh::UniformL1AnnIndex(m, mass, t, eng));
+        } else {
+            run(uh::UniformL2AnnIndex(m, mass, t, eng));
         }
     });
 }

@@ -122,6 +122,61 @@ def test_flat_parity_bit_exact(uh, oracle_mod, metric, r, d, b, T):
     np.testing.assert_array_equal(ddist.view(np.uint32), e_dist.view(np.uint32))
 
 
+@pytest.mark.parametrize("metric", ["l2", "l1"])
+def test_uniform_reference_fixture(uh, oracle_mod, metric):
+    # tests/coordinate_sampling/uniform_ann_index_test.cpp:26-77
+    reps = np.array([[0, 0], [5, 5]], F32)
+    cls = uh.HipUniformL1AnnIndex if metric == "l1" else uh.HipUniformL2AnnIndex
+    queries = np.array([[1, 1], [4, 4]], F32)
+    index = cls(reps, 2.0, 4, uh.Rng(42))
+    ws = index.make_query_workspace(2)
+    assert index.query(queries[0], ws) == 0
+    assert index.query(queries[1], ws) == 1
+    idx, _ = index.query_batch(queries, 2, ws)
+    assert idx.tolist() == [0, 1]
+    usage = index.space_usage()
+    assert usage["unique_query_coordinates"] == 2 and usage["sampled_multiplicity"] == 2 * 4
+    empty = cls(reps, 0.0, 4, uh.Rng(42))
+    ews = empty.make_query_workspace(1)
+    assert empty.query(queries[1], ews) == 0  # all-zero distances tie to row 0
+    eu = empty.space_usage()
+    assert eu["unique_query_coordinates"] == 0 and eu["sampled_multiplicity"] == 0
+    if oracle_mod.ref_available():
+        out = np.zeros(2, np.uint64)
+        usage_ref = np.zeros(2, np.uint64)
+        assert oracle_mod.ref().ref_uniform(int(metric == "l1"), oracle_mod._p(reps, oracle_mod._f32p), 2, 2, 2.0, 4, 42,
+                                            oracle_mod._p(queries, oracle_mod._f32p), 2,
+                                            oracle_mod._p(out, oracle_mod._u64p),
+                                            oracle_mod._p(usage_ref, oracle_mod._u64p)) == 0
+        assert out.tolist() == idx.tolist() and usage_ref.tolist() == [2, 8]
+
+
+@pytest.mark.parametrize("metric", ["l2", "l1"])
+@pytest.mark.parametrize("r,d,b,T", [(60, 900, 17, 16), (300, 4000, 40, 64)])
+def test_uniform_parity_bit_exact(uh, oracle_mod, metric, r, d, b, T):
+    """The uniform kind is the flat scan over the mass-matched uniform
+    sample: same sample as the oracle's draw from mass / d, bit-identical
+    distances to the oracle's K2 / weighted K5 emulation."""
+    refs, q = _random_index_case(r * 3 + d, r, d, b)
+    l1 = metric == "l1"
+    mass = uh.sampling_mass(oracle_mod.importance_probabilities(l1, refs))
+    cls = uh.HipUniformL1AnnIndex if l1 else uh.HipUniformL2AnnIndex
+    index = cls(refs, mass, T, uh.Rng(1312))
+    ws = index.make_query_workspace(b)
+    cols, mult, invp = oracle_mod.build_coordinate_sample(np.full(d, mass / d), T, 1312)
+    np.testing.assert_array_equal(index.sample["source_column"], cols)
+    np.testing.assert_array_equal(index.sample["multiplicity"], mult)
+    scales, weights = oracle_mod.embedding_factors(mult, invp)
+    if l1:
+        emul = oracle_mod.k5_distances(_emb(refs, cols), _emb(q, cols), weights, 0)
+    else:
+        emul = oracle_mod.k2_distances(_emb(refs, cols, scales), _emb(q, cols, scales))
+    e_idx, e_dist = oracle_mod.fp32_argmin(emul)
+    idx, dist = index.query_batch(q, b, ws)
+    np.testing.assert_array_equal(idx, e_idx)
+    np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
+
+
 def test_flat_l2_mixed_matches_direct(uh, oracle_mod):
     refs, q = _random_index_case(5, 400, 6000, 30)
     probs = oracle_mod.importance_probabilities(False, refs)

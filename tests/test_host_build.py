@@ -111,3 +111,52 @@ def test_rng_equality_and_draws():
     assert a == b
     # std::mt19937_64 with the default seed 5489 yields 14514284786278117030 first
     assert uh.Rng(5489).next() == 14514284786278117030
+
+
+def test_uniform_probabilities_reference_fixture():
+    # tests/coordinate_sampling/uniform_probabilities_test.cpp:26-86
+    importance = np.array([0.5, 0.25, 0.0, 0.25])
+    mass = uh.sampling_mass(importance)
+    assert abs(mass - 1.0) < 1e-12
+    uniform = uh.uniform_probabilities(importance.size, mass)
+    assert uniform.shape == (4,)
+    assert np.all(np.abs(uniform - 0.25) < 1e-12)
+    assert abs(uh.sampling_mass(uniform) - mass) < 1e-12
+    with pytest.raises(uh.UhannDomainError):
+        uh.sampling_mass(np.array([np.nan]))
+    with pytest.raises(uh.UhannDomainError):
+        uh.uniform_probabilities(2, 3.0)
+    # the remaining domain checks of the same functions
+    with pytest.raises(uh.UhannDomainError):
+        uh.sampling_mass(np.array([0.5, 1.5]))
+    with pytest.raises(uh.UhannDomainError):
+        uh.sampling_mass(np.array([-0.25]))
+    with pytest.raises(uh.UhannDomainError):
+        uh.uniform_probabilities(3, -1.0)
+    with pytest.raises(uh.UhannDomainError):
+        uh.uniform_probabilities(3, np.inf)
+    assert uh.uniform_probabilities(0, 0.0).size == 0
+    assert uh.uniform_probabilities(5, 5.0).tolist() == [1.0] * 5
+
+
+def test_uniform_probabilities_match_reference(ref_mod):
+    """Sequential-sum mass and mass / d, bit for bit with the reference's own
+    functions on importance probabilities of random data."""
+    g = np_rng(77)
+    refs = (g.random((40, 3000), dtype=np.float32) * 3).astype(np.float32)
+    for metric_l1 in (False, True):
+        probs = ref_mod.importance_probabilities(metric_l1, refs)
+        mine = uh.sampling_mass(probs)
+        theirs = np.zeros(1, np.float64)
+        assert ref_mod.ref().ref_sampling_mass(ref_mod._p(probs, ref_mod._f64p), probs.size,
+                                               ref_mod._p(theirs, ref_mod._f64p)) == 0
+        assert mine == theirs[0]
+        # a different summation order would change the last bits; the sum in order is
+        # also what numpy's sequential cumsum gives
+        assert mine == np.cumsum(probs)[-1]
+        u_mine = uh.uniform_probabilities(probs.size, mine)
+        u_theirs = np.zeros(probs.size, np.float64)
+        assert ref_mod.ref().ref_uniform_probabilities(probs.size, mine, ref_mod._p(u_theirs, ref_mod._f64p)) == 0
+        np.testing.assert_array_equal(u_mine, u_theirs)
+    bad = np.zeros(1, np.float64)
+    assert ref_mod.ref().ref_uniform_probabilities(2, 3.0, ref_mod._p(bad, ref_mod._f64p)) == 1

@@ -44,12 +44,16 @@ from .indexes import (  # noqa: E402
     HipHierarchicalL2AnnIndex,
     HipSampledCoordinateL1AnnIndex,
     HipSampledCoordinateL2AnnIndex,
+    HipUniformL1AnnIndex,
+    HipUniformL2AnnIndex,
     Rng,
     build_coordinate_sample,
     build_projection,
     importance_probabilities,
     importance_probabilities_device,
     make_sample,
+    sampling_mass,
+    uniform_probabilities,
 )
 from .serving import INDEX_FAILED, INDEX_PENDING, INDEX_RESIDENT, Server  # noqa: E402
 
@@ -75,7 +79,11 @@ __all__ = [
     "HipHierarchicalL2AnnIndex",
     "HipSampledCoordinateL1AnnIndex",
     "HipSampledCoordinateL2AnnIndex",
+    "HipUniformL1AnnIndex",
+    "HipUniformL2AnnIndex",
     "Rng",
+    "sampling_mass",
+    "uniform_probabilities",
     "build_coordinate_sample",
     "build_projection",
     "importance_probabilities",

@@ -107,6 +107,23 @@ int uhann_build_coordinate_sample(uhann_rng* rng, const double* probabilities, u
     });
 }
 
+int uhann_sampling_mass(const double* probabilities, uint64_t dim, double* mass)
+{
+    return guarded([&] {
+        if (mass == nullptr || (dim != 0 && probabilities == nullptr)) throw std::invalid_argument("null argument");
+        *mass = uab::sampling_mass(std::span<const double>(probabilities, dim));
+    });
+}
+
+int uhann_uniform_probabilities(uint64_t dim, double mass, double* out)
+{
+    return guarded([&] {
+        if (dim != 0 && out == nullptr) throw std::invalid_argument("output must not be null");
+        const auto p = uab::uniform_probabilities(dim, mass);
+        std::copy(p.begin(), p.end(), out);
+    });
+}
+
 int uhann_build_projection(uhann_rng* rng, int metric, const uhann_sampled_column* sample,
                            uint64_t count, uint64_t projection_dim, double* out)
 {

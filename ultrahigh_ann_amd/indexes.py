@@ -87,6 +87,22 @@ def build_projection(sample: np.ndarray, projection_dimension: int, metric: int,
     return out
 
 
+def sampling_mass(probabilities) -> float:
+    """Reference compute_sampling_mass (uniform_probabilities.cpp:12-26)."""
+    p = np.ascontiguousarray(probabilities, dtype=np.float64)
+    out = ctypes.c_double()
+    check(_native.lib().uhann_sampling_mass(f64ptr(p), p.size, ctypes.byref(out)))
+    return out.value
+
+
+def uniform_probabilities(dimension: int, mass: float) -> np.ndarray:
+    """Reference make_mass_matched_uniform_probabilities
+    (uniform_probabilities.cpp:28-45)."""
+    out = np.zeros(max(0, int(dimension)), np.float64)
+    check(_native.lib().uhann_uniform_probabilities(int(dimension), float(mass), f64ptr(out)))
+    return out
+
+
 def importance_probabilities(metric: int, representatives) -> np.ndarray:
     refs = np.ascontiguousarray(representatives, dtype=np.float32)
     out = np.zeros(refs.shape[1], np.float64)
@@ -301,6 +317,28 @@ class HipFlatL1AnnIndex(HipSampledCoordinateL1AnnIndex):
     def __init__(self, input_matrix, probabilities, repetitions: int, rng: Rng, device: int = 0):
         refs = _refs(input_matrix)
         super().__init__(refs, _flat_sample(refs, probabilities, repetitions, rng), device)
+
+
+class HipUniformL2AnnIndex(HipSampledCoordinateL2AnnIndex):
+    """Reference UniformL2AnnIndex(input, sampling_mass, T, engine)
+    (uniform_l2_ann_index.cpp:11-21): the mass-matched uniform probabilities'
+    coordinate sample, scanned on the GPU as the reference's benchmark does
+    (nearest_neighbor_benchmark.cpp:755-768)."""
+
+    def __init__(self, input_matrix, sampling_mass: float, repetitions: int, rng: Rng, device: int = 0):
+        refs = _refs(input_matrix)
+        probs = uniform_probabilities(refs.shape[1], sampling_mass)
+        super().__init__(refs, build_coordinate_sample(probs, repetitions, rng), device)
+
+
+class HipUniformL1AnnIndex(HipSampledCoordinateL1AnnIndex):
+    """Reference UniformL1AnnIndex(input, sampling_mass, T, engine)
+    (uniform_l1_ann_index.cpp:11-23), scanned on the GPU."""
+
+    def __init__(self, input_matrix, sampling_mass: float, repetitions: int, rng: Rng, device: int = 0):
+        refs = _refs(input_matrix)
+        probs = uniform_probabilities(refs.shape[1], sampling_mass)
+        super().__init__(refs, build_coordinate_sample(probs, repetitions, rng), device)
 
 
 class _Hierarchical(_Index):
