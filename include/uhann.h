@@ -319,6 +319,29 @@ int uhann_importance_probabilities(int metric, const float* refs, uint64_t rows,
 int uhann_importance_probabilities_device(int metric, const float* host_refs, uint64_t rows,
                                           uint64_t dim, int device, double* out);
 
+/* ==== importance-probability files (the reference's UAP v2 format) ======
+ * io/importance_probability_io.cpp and io/sha256.hpp; layout and checks in
+ * include/ultrahigh_ann_amd/probability_file.hpp. File and format errors ->
+ * RUNTIME with the reference's message ("invalid importance-probability
+ * file <path>: <reason>"). */
+
+/* sha256_file: SHA-256 of the file's exact bytes into digest[32]. */
+int uhann_sha256_file(const char* path, uint8_t* digest);
+
+/* save_importance_probabilities (importance_probability_io.cpp:124-162):
+ * metric UHANN_METRIC_L1 or UHANN_METRIC_L2 (stored as kind 1 / 2),
+ * representative count, the representative matrix file's digest[32] and dim
+ * probabilities (finite, in [0, 1]). Creates parent directories. */
+int uhann_save_probabilities(const char* path, int metric, uint64_t representative_count,
+                             const uint8_t* digest, const double* probabilities, uint64_t dim);
+
+/* load_importance_probabilities (importance_probability_io.cpp:164-227):
+ * header into *metric, *representative_count and digest[32], the coordinate
+ * count into *dim and the values into out. With out == NULL only the header
+ * and *dim are returned; capacity < *dim -> LENGTH. */
+int uhann_load_probabilities(const char* path, int* metric, uint64_t* representative_count, uint8_t* digest,
+                             double* out, uint64_t capacity, uint64_t* dim);
+
 /* The mixed strategy's Gram estimates ||q||^2 + ||v||^2 - 2 <bf16(q), bf16(v)>
  * (binary32 MFMA accumulation) for count device queries against every
  * reference, written to device_out (count rows of ld_out >= rows floats) on

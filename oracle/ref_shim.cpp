@@ -22,6 +22,8 @@
 #include "ultrahigh_ann/hnsvw25/l2/flat/l2_ann_index.hpp"
 #include "ultrahigh_ann/hnsvw25/l2/hierarchical/hierarchical_l2_ann_index.hpp"
 #include "ultrahigh_ann/hnsvw25/l2/importance_sampling.hpp"
+#include "ultrahigh_ann/io/importance_probability_io.hpp"
+#include "ultrahigh_ann/io/sha256.hpp"
 
 #include "hnsvw25/l1/hierarchical/l1_hierarchical_projection.hpp"
 #include "hnsvw25/l2/hierarchical/l2_hierarchical_projection.hpp"
@@ -159,6 +161,42 @@ int ref_uniform(int metric_l1, const float* refs, uint64_t r, uint64_t d, double
         } else {
             run(uh::UniformL2AnnIndex(m, mass, t, eng));
         }
+    });
+}
+
+// io::sha256_file into digest[32].
+int ref_sha256_file(const char* path, uint8_t* digest)
+{
+    return guarded([&] {
+        const auto d = uh::io::sha256_file(path);
+        std::memcpy(digest, d.data(), d.size());
+    });
+}
+
+// io::save_importance_probabilities (kind_l1: 1 = l1, 0 = l2).
+int ref_save_probabilities(const char* path, int kind_l1, uint64_t rows, const uint8_t* digest,
+                           const double* probs, uint64_t d)
+{
+    return guarded([&] {
+        uh::io::Sha256Digest dg{};
+        std::memcpy(dg.data(), digest, dg.size());
+        uh::io::save_importance_probabilities(
+            path, kind_l1 ? uh::io::ImportanceProbabilityKind::l1 : uh::io::ImportanceProbabilityKind::l2, rows, dg,
+            std::span<const double>(probs, d));
+    });
+}
+
+// io::load_importance_probabilities; out holds capacity doubles.
+int ref_load_probabilities(const char* path, int* kind_l1, uint64_t* rows, uint8_t* digest, double* out,
+                           uint64_t capacity, uint64_t* d)
+{
+    return guarded([&] {
+        const auto f = uh::io::load_importance_probabilities(path);
+        *kind_l1 = f.kind == uh::io::ImportanceProbabilityKind::l1 ? 1 : 0;
+        *rows = f.representative_count;
+        std::memcpy(digest, f.representatives_sha256.data(), 32);
+        *d = f.probabilities.size();
+        for (std::size_t i = 0; i < f.probabilities.size() && i < capacity; ++i) out[i] = f.probabilities[i];
     });
 }
 

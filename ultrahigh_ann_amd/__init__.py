@@ -51,8 +51,13 @@ from .indexes import (  # noqa: E402
     build_projection,
     importance_probabilities,
     importance_probabilities_device,
+    ProbabilityFile,
+    load_importance_probabilities,
     make_sample,
+    probability_source_matches,
     sampling_mass,
+    save_importance_probabilities,
+    sha256_file,
     uniform_probabilities,
 )
 from .serving import INDEX_FAILED, INDEX_PENDING, INDEX_RESIDENT, Server  # noqa: E402
@@ -84,6 +89,11 @@ __all__ = [
     "Rng",
     "sampling_mass",
     "uniform_probabilities",
+    "ProbabilityFile",
+    "load_importance_probabilities",
+    "probability_source_matches",
+    "save_importance_probabilities",
+    "sha256_file",
     "build_coordinate_sample",
     "build_projection",
     "importance_probabilities",

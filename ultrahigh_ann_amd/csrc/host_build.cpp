@@ -16,6 +16,7 @@
 #include "engine.hpp"
 #include "probabilities.hpp"
 #include "ultrahigh_ann_amd/build.hpp"
+#include "ultrahigh_ann_amd/probability_file.hpp"
 
 namespace uab = ultrahigh_ann_amd;
 using uhann::guarded;
@@ -121,6 +122,51 @@ int uhann_uniform_probabilities(uint64_t dim, double mass, double* out)
         if (dim != 0 && out == nullptr) throw std::invalid_argument("output must not be null");
         const auto p = uab::uniform_probabilities(dim, mass);
         std::copy(p.begin(), p.end(), out);
+    });
+}
+
+int uhann_sha256_file(const char* path, uint8_t* digest)
+{
+    return guarded([&] {
+        if (path == nullptr || digest == nullptr) throw std::invalid_argument("null argument");
+        const auto d = uab::sha256_file(path);
+        std::memcpy(digest, d.data(), d.size());
+    });
+}
+
+int uhann_save_probabilities(const char* path, int metric, uint64_t representative_count,
+                             const uint8_t* digest, const double* probabilities, uint64_t dim)
+{
+    return guarded([&] {
+        if (path == nullptr || digest == nullptr || (dim != 0 && probabilities == nullptr)) {
+            throw std::invalid_argument("null argument");
+        }
+        if (metric != UHANN_METRIC_L2 && metric != UHANN_METRIC_L1) {
+            throw std::invalid_argument("unknown importance-probability kind");
+        }
+        uab::Sha256Digest d{};
+        std::memcpy(d.data(), digest, d.size());
+        uab::save_probabilities(path, metric == UHANN_METRIC_L1 ? uab::ProbabilityKind::l1 : uab::ProbabilityKind::l2,
+                                representative_count, d, std::span<const double>(probabilities, dim));
+    });
+}
+
+int uhann_load_probabilities(const char* path, int* metric, uint64_t* representative_count, uint8_t* digest,
+                             double* out, uint64_t capacity, uint64_t* dim)
+{
+    return guarded([&] {
+        if (path == nullptr || metric == nullptr || representative_count == nullptr || digest == nullptr ||
+            dim == nullptr) {
+            throw std::invalid_argument("null argument");
+        }
+        const auto f = uab::load_probabilities(path);
+        *metric = f.kind == uab::ProbabilityKind::l1 ? UHANN_METRIC_L1 : UHANN_METRIC_L2;
+        *representative_count = f.representative_count;
+        std::memcpy(digest, f.representatives_sha256.data(), f.representatives_sha256.size());
+        *dim = f.probabilities.size();
+        if (out == nullptr) return;
+        if (capacity < f.probabilities.size()) throw std::length_error("probability output capacity is too small");
+        std::copy(f.probabilities.begin(), f.probabilities.end(), out);
     });
 }
 

@@ -11,6 +11,7 @@ samples (hnsvw25/l2/flat/cuda_l2_ann_index.cpp:13-37).
 from __future__ import annotations
 
 import ctypes
+import os
 from ctypes import c_int, c_uint64, c_void_p
 
 import numpy as np
@@ -101,6 +102,59 @@ def uniform_probabilities(dimension: int, mass: float) -> np.ndarray:
     out = np.zeros(max(0, int(dimension)), np.float64)
     check(_native.lib().uhann_uniform_probabilities(int(dimension), float(mass), f64ptr(out)))
     return out
+
+
+def _path(path) -> bytes:
+    return os.fsencode(os.fspath(path))
+
+
+def sha256_file(path) -> bytes:
+    """SHA-256 of a file's bytes (reference io::sha256_file)."""
+    out = ctypes.create_string_buffer(32)
+    check(_native.lib().uhann_sha256_file(_path(path), out))
+    return out.raw
+
+
+class ProbabilityFile:
+    """Contents of an importance-probability file (reference
+    io::ImportanceProbabilityFile)."""
+
+    def __init__(self, metric: int, representative_count: int, sha256: bytes, probabilities: np.ndarray):
+        self.metric = metric
+        self.representative_count = representative_count
+        self.sha256 = sha256
+        self.probabilities = probabilities
+
+
+def save_importance_probabilities(path, metric: int, representative_count: int, sha256: bytes,
+                                  probabilities) -> None:
+    """Reference io::save_importance_probabilities (UAP v2 file)."""
+    p = np.ascontiguousarray(probabilities, dtype=np.float64)
+    digest = bytes(sha256)
+    if len(digest) != 32:
+        raise UhannInvalidArgument("the representative digest must be 32 bytes")
+    check(_native.lib().uhann_save_probabilities(_path(path), metric, int(representative_count),
+                                                 ctypes.create_string_buffer(digest, 32), f64ptr(p), p.size))
+
+
+def load_importance_probabilities(path) -> ProbabilityFile:
+    """Reference io::load_importance_probabilities."""
+    metric, count, dim = c_int(), c_uint64(), c_uint64()
+    digest = ctypes.create_string_buffer(32)
+    lib = _native.lib()
+    check(lib.uhann_load_probabilities(_path(path), ctypes.byref(metric), ctypes.byref(count), digest, None, 0,
+                                       ctypes.byref(dim)))
+    out = np.zeros(dim.value, np.float64)
+    check(lib.uhann_load_probabilities(_path(path), ctypes.byref(metric), ctypes.byref(count), digest,
+                                       f64ptr(out), out.size, ctypes.byref(dim)))
+    return ProbabilityFile(metric.value, count.value, digest.raw, out)
+
+
+def probability_source_matches(f: ProbabilityFile, metric: int, representative_count: int,
+                               coordinate_count: int, sha256: bytes) -> bool:
+    """Reference io::importance_probability_source_matches."""
+    return (f.metric == metric and f.representative_count == representative_count
+            and f.probabilities.size == coordinate_count and f.sha256 == bytes(sha256))
 
 
 def importance_probabilities(metric: int, representatives) -> np.ndarray:
