@@ -356,6 +356,16 @@ int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const
                               uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out,
                               int tile, void* stream);
 
+/* Exact diagnostic distances for count device queries against every
+ * reference of an unweighted handle, binary64 in the reference's CPU order
+ * (L2: sqrt of the sequential sum of squared differences, L1: the
+ * transform_reduce sum), written to device_out (count rows of ld_out >= rows
+ * doubles) on `stream`. Replaces the O(B r d) loop of the reference's
+ * ExactDistanceTable constructor (benchmarks/benchmark_metrics.cpp:112-192,
+ * exact_distance :32-41); the statistics over the table stay on the host. */
+int uhann_scan_distance_table(const uhann_scan* scan, const float* device_queries, uint64_t ld_queries,
+                              uint64_t count, double* device_out, uint64_t ld_out, void* stream);
+
 /* The GRAM_FP8 strategy's estimates ||q||^2 + ||v||^2 - 2 s_q s_v <q8, v8>
  * for count device queries against every reference, as
  * uhann_scan_gram_estimates. Needs uhann_scan_enable_mixed(GRAM_FP8). No

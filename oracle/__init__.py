@@ -106,6 +106,8 @@ def ref():
                                              c_uint64, _u64p, _u64p, _f64p, _f32p, _f32p], c_int64),
             "ref_importance_probabilities": ([c_int, _f32p, c_uint64, c_uint64, _f64p], c_int),
             "ref_sampling_mass": ([_f64p, c_uint64, _f64p], c_int),
+            "ref_distance_diagnostics": ([c_int, _f32p, c_uint64, c_uint64, _f32p, c_uint64, _u64p, _u64p, _f64p,
+                                          _f64p, _f64p], c_int),
             "ref_sha256_file": ([ctypes.c_char_p, ctypes.c_void_p], c_int),
             "ref_save_probabilities": ([ctypes.c_char_p, c_int, c_uint64, ctypes.c_void_p, _f64p, c_uint64], c_int),
             "ref_load_probabilities": ([ctypes.c_char_p, POINTER(c_int), _u64p, ctypes.c_void_p, _f64p, c_uint64,

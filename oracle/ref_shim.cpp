@@ -25,11 +25,14 @@
 #include "ultrahigh_ann/io/importance_probability_io.hpp"
 #include "ultrahigh_ann/io/sha256.hpp"
 
+#include "benchmarks/benchmark_metrics.hpp"
 #include "hnsvw25/l1/hierarchical/l1_hierarchical_projection.hpp"
 #include "hnsvw25/l2/hierarchical/l2_hierarchical_projection.hpp"
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <exception>
 #include <random>
 #include <span>
@@ -55,9 +58,94 @@ template <class F> int guarded(F&& body)
     }
 }
 
+namespace bm = ultrahigh_ann::benchmark;
+
+constexpr double kNone = std::numeric_limits<double>::quiet_NaN();
+
+double* put_summary(double* o, const bm::DistributionSummary& s)
+{
+    *o++ = static_cast<double>(s.count);
+    *o++ = s.mean.value_or(kNone);
+    *o++ = s.median.value_or(kNone);
+    *o++ = s.percentile_95.value_or(kNone);
+    *o++ = s.percentile_99.value_or(kNone);
+    *o++ = s.maximum.value_or(kNone);
+    return o;
+}
+
+// Five margin buckets of 6 values (lower, upper, queries, non-optimal,
+// ratio-eligible, mean ratio); NaN where a bucket list is empty.
+double* put_buckets(double* o, const std::vector<bm::MarginBucketMetrics>& buckets)
+{
+    for (std::size_t i = 0; i < 5; ++i) {
+        if (i < buckets.size()) {
+            const auto& b = buckets[i];
+            *o++ = b.lower_inclusive;
+            *o++ = b.upper_exclusive.value_or(kNone);
+            *o++ = static_cast<double>(b.query_count);
+            *o++ = static_cast<double>(b.non_optimal_count);
+            *o++ = static_cast<double>(b.ratio_eligible_count);
+            *o++ = b.mean_distance_ratio.value_or(kNone);
+        } else {
+            for (int k = 0; k < 6; ++k) *o++ = kNone;
+        }
+    }
+    return o;
+}
+
+// 63 values: 9 counts / means, 6 x (epsilon, violations), the distance-ratio
+// and rank summaries, the buckets.
+void put_metrics(double* o, const bm::ApproximationMetrics& m)
+{
+    *o++ = static_cast<double>(m.query_count);
+    *o++ = static_cast<double>(m.optimal_representative_count);
+    *o++ = static_cast<double>(m.non_optimal_count);
+    *o++ = static_cast<double>(m.reference_improvement_count);
+    *o++ = static_cast<double>(m.zero_optimum_query_count);
+    *o++ = static_cast<double>(m.zero_optimum_non_optimal_count);
+    *o++ = static_cast<double>(m.conditional_non_optimal_ratio_count);
+    *o++ = m.conditional_mean_distance_ratio.value_or(kNone);
+    *o++ = m.conditional_mean_relative_excess.value_or(kNone);
+    for (const auto& f : m.approximation_failures) {
+        *o++ = f.epsilon;
+        *o++ = static_cast<double>(f.violation_count);
+    }
+    o = put_summary(o, m.distance_ratio);
+    o = put_summary(o, m.returned_representative_rank);
+    put_buckets(o, m.margin_buckets);
+}
+
 }  // namespace
 
 extern "C" {
+
+// benchmark::ExactDistanceTable(metric, refs, queries, b): query_geometry()
+// into geo[40] (4 counts, the finite-margin summary, the buckets),
+// evaluate(pred) into ev[63] and evaluate_selected_distances(ref_pred, pred)
+// into sel[63].
+int ref_distance_diagnostics(int metric_l1, const float* refs, uint64_t r, uint64_t d, const float* queries,
+                             uint64_t b, const uint64_t* pred, const uint64_t* ref_pred, double* geo, double* ev,
+                             double* sel)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        const auto q = make_matrix(queries, b, d);
+        const auto metric = metric_l1 ? bm::DistanceMetric::l1 : bm::DistanceMetric::l2;
+        const bm::ExactDistanceTable table(metric, m, q, b);
+        const auto& g = table.query_geometry();
+        double* o = geo;
+        *o++ = static_cast<double>(g.query_count);
+        *o++ = static_cast<double>(g.zero_optimum_query_count);
+        *o++ = static_cast<double>(g.non_unique_optimum_query_count);
+        *o++ = static_cast<double>(g.infinite_multiplicative_margin_count);
+        o = put_summary(o, g.finite_multiplicative_margin);
+        put_buckets(o, g.margin_buckets);
+        std::vector<std::size_t> p(pred, pred + b), rp(ref_pred, ref_pred + b);
+        put_metrics(ev, table.evaluate(p));
+        put_metrics(sel, bm::evaluate_selected_distances(metric, m, q, b, rp, p));
+    });
+}
+
 
 // ExactL2Index::query_batch with CpuDenseL2QueryStrategy: 0 sequential,
 // 2 parallel_queries (dense_l2_scan_index.hpp enum order checked in tests).

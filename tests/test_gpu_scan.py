@@ -218,3 +218,41 @@ def test_direct_kernel_timing_keeps_results(uh, oracle_mod):
     assert times[0] > 0.0 and times[1:] == [0.0, 0.0, 0.0]
     exp_idx, _ = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q))
     np.testing.assert_array_equal(plain[0], exp_idx)
+
+
+@pytest.mark.parametrize("metric_l1", [False, True])
+@pytest.mark.parametrize("r,d,b", [(7, 33, 5), (130, 1000, 70), (65, 4097, 129)])
+def test_distance_table_bit_exact(uh, oracle_mod, metric_l1, r, d, b):
+    """The GPU diagnostic table is the reference CPU order's binary64
+    distance, bit for bit (L2 with a correctly rounded square root), for
+    ragged tiles; the diagnostics over it equal the same computed from the
+    oracle's table."""
+    import torch
+
+    g = np.random.default_rng(r * d + b)
+    refs = (g.random((r, d), dtype=np.float32) * 3).astype(np.float32)
+    q = (g.random((b, d), dtype=np.float32) * 3).astype(np.float32)
+    q[0] = refs[r // 2]  # a zero distance
+    index = uh.HipExactL1Index(refs) if metric_l1 else uh.HipExactL2Index(refs)
+    dq = torch.from_numpy(q).cuda()
+    out = torch.full((b, r + 2), float("nan"), dtype=torch.float64, device="cuda")
+    index.distance_table(dq, b, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.all(np.isnan(got[:, r:]))
+    t = oracle_mod.fp64_table(metric_l1, refs, q)
+    want = t if metric_l1 else np.sqrt(t)
+    np.testing.assert_array_equal(got[:, :r].view(np.uint64), want.view(np.uint64))
+    table = uh.ExactDistanceTable(index, dq, b, chunk_rows=32)
+    np.testing.assert_array_equal(table.distances.view(np.uint64), want.view(np.uint64))
+    pred = np.asarray(index.query_batch(q, b, index.make_query_workspace(b))[0])
+    assert table.evaluate(pred) == uh.ExactDistanceTable.from_table(want).evaluate(pred)
+
+
+def test_distance_table_rejects_weighted(uh):
+    import torch
+
+    index = uh.HipDenseL1ScanIndex(np.ones((3, 4), np.float32), weights=np.ones(4, np.float32))
+    out = torch.zeros((1, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(uh.UhannInvalidArgument):
+        index.distance_table(torch.ones((1, 4), device="cuda"), 1, out)

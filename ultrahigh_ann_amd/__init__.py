@@ -61,6 +61,7 @@ from .indexes import (  # noqa: E402
     uniform_probabilities,
 )
 from .serving import INDEX_FAILED, INDEX_PENDING, INDEX_RESIDENT, Server  # noqa: E402
+from .diagnostics import ExactDistanceTable, evaluate_selected_distances  # noqa: E402
 
 METRIC_L2 = 0
 METRIC_L1 = 1
@@ -100,6 +101,8 @@ __all__ = [
     "importance_probabilities_device",
     "make_sample",
     "Server",
+    "ExactDistanceTable",
+    "evaluate_selected_distances",
     "INDEX_FAILED",
     "INDEX_PENDING",
     "INDEX_RESIDENT",
@@ -337,6 +340,23 @@ class _DenseScan:
             )
         )
 
+
+    def distance_table(self, device_queries, count: int, out, stream=None, ld_queries=None):
+        """Exact binary64 diagnostic distances (L2: sqrt of the squared
+        distance, L1: the L1 distance; the reference CPU order) of count
+        device queries against every reference into the float64 device
+        tensor out (count x rows, row stride out.stride(0))."""
+        import torch
+
+        if out.dtype != torch.float64:
+            raise UhannInvalidArgument("the distance table is float64")
+        ld = ld_queries if ld_queries is not None else self.dim
+        check(
+            _native.lib().uhann_scan_distance_table(
+                self._h, c_void_p(device_queries.data_ptr()), ld, count, c_void_p(out.data_ptr()),
+                out.stride(0), _torch_stream_ptr(stream),
+            )
+        )
 
     def fp8_estimates(self, device_queries, count: int, workspace: QueryWorkspace, out, stream=None,
                       ld_queries=None):

@@ -147,6 +147,7 @@ def _declare(lib):
         "uhann_importance_probabilities": (c_int, [c_int, f32p, c_uint64, c_uint64, f64p]),
         "uhann_sampling_mass": (c_int, [f64p, c_uint64, f64p]),
         "uhann_sha256_file": (c_int, [c_char_p, c_void_p]),
+        "uhann_scan_distance_table": (c_int, [c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_uint64, c_void_p]),
         "uhann_save_probabilities": (c_int, [c_char_p, c_int, c_uint64, c_void_p, f64p, c_uint64]),
         "uhann_load_probabilities": (c_int, [c_char_p, POINTER(c_int), u64p, c_void_p, f64p, c_uint64, u64p]),
         "uhann_uniform_probabilities": (c_int, [c_uint64, c_double, f64p]),

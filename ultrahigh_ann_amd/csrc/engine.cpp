@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
+#include "probabilities.hpp"
 
 namespace uhann {
 
@@ -814,6 +815,29 @@ int uhann_scan_gram_estimates(const uhann_scan* scan, uhann_workspace* ws, const
                         static_cast<uint32_t>(g.ldk), ws->qstats.data(), g.stats.data(), device_out, ld_out,
                         tile < 0 ? gram_tile() : tile, tile < 0 ? ws->gram_partials.data() : nullptr,
                         ws->gram_partials.bytes() / sizeof(float), s);
+    });
+}
+
+int uhann_scan_distance_table(const uhann_scan* scan, const float* device_queries, uint64_t ld_queries,
+                              uint64_t count, double* device_out, uint64_t ld_out, void* stream)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        if (scan->weighted) throw std::invalid_argument("distance tables are defined for unweighted references");
+        if (count == 0 || scan->rows == 0) return;
+        if (device_queries == nullptr || device_out == nullptr) {
+            throw std::invalid_argument("device pointers must not be null");
+        }
+        if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
+        if (ld_out < scan->rows) throw std::invalid_argument("output row stride is below the reference count");
+        if (count > 0xffffffffULL || scan->rows > 0xffffffffULL || scan->dim > 0xffffffffULL) {
+            throw std::length_error("distance table dimensions exceed 32-bit indexing");
+        }
+        select_device(scan->device);
+        kern::distance_table(scan->metric == UHANN_METRIC_L1, device_queries, ld_queries,
+                             static_cast<uint32_t>(count), scan->refs.data(), scan->ld,
+                             static_cast<uint32_t>(scan->rows), static_cast<uint32_t>(scan->dim), device_out, ld_out,
+                             static_cast<hipStream_t>(stream));
     });
 }
 

@@ -43,29 +43,24 @@ void check_launch(const char* what)
     }
 }
 
+// The binary64 distances of a 64 x 64 tile of pairs (rows a0.. of A against
+// rows b0.. of B), in the CPU order: each thread's 4 x 4 pairs sum their
+// columns strictly in order (c = 0, 1, ...) with separate multiply and add
+// (L2: (a - b)^2; L1: transform_reduce's init + ((t0 + t1) + (t2 + t3)) per
+// complete group of four, then the tail term by term). Rows past the end
+// repeat the last row (their sums are not used). Thread (ti, tj) =
+// (tid / 16, tid % 16) owns pairs (ti*4 + i, tj*4 + j).
 template <bool L1>
-__global__ void __launch_bounds__(kPairThreads)
-pair_inverse_kernel(const float* __restrict__ refs, uint64_t ld, uint32_t rows, uint32_t dim,
-                    double* __restrict__ inv, uint32_t tiles)
+__device__ __forceinline__ void pair_tile_fp64(const float* __restrict__ A, uint64_t lda, uint32_t rows_a, uint32_t a0,
+                                               const float* __restrict__ B, uint64_t ldb, uint32_t rows_b, uint32_t b0,
+                                               uint32_t dim, double (&acc)[4][4])
 {
-    // Upper-triangular tile index -> (ta, tb) with ta <= tb.
-    uint32_t t = blockIdx.x;
-    uint32_t ta = 0;
-    while (t >= tiles - ta) {
-        t -= tiles - ta;
-        ++ta;
-    }
-    const uint32_t tb = ta + t;
-    const uint32_t a0 = ta * kPairTile;
-    const uint32_t b0 = tb * kPairTile;
-
     __shared__ double sa[kPairTile * kLdsStride];
     __shared__ double sb[kPairTile * kLdsStride];
     const int tid = threadIdx.x;
     const int ti = tid / 16;  // 16 x 16 threads, 4 x 4 pairs each
     const int tj = tid % 16;
 
-    double acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -80,11 +75,11 @@ pair_inverse_kernel(const float* __restrict__ refs, uint64_t ld, uint32_t rows, 
         for (int e = tid; e < kPairTile * kChunk; e += kPairThreads) {
             const int row = e / kChunk;
             const int col = e % kChunk;
-            const uint32_t ra = min(a0 + row, rows - 1U);
-            const uint32_t rb = min(b0 + row, rows - 1U);
+            const uint32_t ra = min(a0 + row, rows_a - 1U);
+            const uint32_t rb = min(b0 + row, rows_b - 1U);
             const bool in = static_cast<uint32_t>(col) < n;
-            sa[row * kLdsStride + col] = in ? static_cast<double>(refs[static_cast<uint64_t>(ra) * ld + c0 + col]) : 0.0;
-            sb[row * kLdsStride + col] = in ? static_cast<double>(refs[static_cast<uint64_t>(rb) * ld + c0 + col]) : 0.0;
+            sa[row * kLdsStride + col] = in ? static_cast<double>(A[static_cast<uint64_t>(ra) * lda + c0 + col]) : 0.0;
+            sb[row * kLdsStride + col] = in ? static_cast<double>(B[static_cast<uint64_t>(rb) * ldb + c0 + col]) : 0.0;
         }
         __syncthreads();
         if constexpr (!L1) {
@@ -137,6 +132,27 @@ pair_inverse_kernel(const float* __restrict__ refs, uint64_t ld, uint32_t rows, 
             }
         }
     }
+}
+
+template <bool L1>
+__global__ void __launch_bounds__(kPairThreads)
+pair_inverse_kernel(const float* __restrict__ refs, uint64_t ld, uint32_t rows, uint32_t dim,
+                    double* __restrict__ inv, uint32_t tiles)
+{
+    // Upper-triangular tile index -> (ta, tb) with ta <= tb.
+    uint32_t t = blockIdx.x;
+    uint32_t ta = 0;
+    while (t >= tiles - ta) {
+        t -= tiles - ta;
+        ++ta;
+    }
+    const uint32_t tb = ta + t;
+    const uint32_t a0 = ta * kPairTile;
+    const uint32_t b0 = tb * kPairTile;
+    double acc[4][4];
+    pair_tile_fp64<L1>(refs, ld, rows, a0, refs, ld, rows, b0, dim, acc);
+    const int ti = threadIdx.x / 16;
+    const int tj = threadIdx.x % 16;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -147,6 +163,31 @@ pair_inverse_kernel(const float* __restrict__ refs, uint64_t ld, uint32_t rows, 
                 const double dist = acc[i][j];
                 inv[static_cast<uint64_t>(a) * rows + b] = dist == 0.0 ? 0.0 : 1.0 / dist;
             }
+        }
+    }
+}
+
+// out[q][r] = the reference's exact diagnostic distance of query q and
+// reference r (benchmark_metrics.cpp:32-41): L2 sqrt(squared_l2_dist), L1
+// l1_dist, both in the CPU order above; sqrt correctly rounded.
+template <bool L1>
+__global__ void __launch_bounds__(kPairThreads)
+distance_table_kernel(const float* __restrict__ Q, uint64_t ldq, uint32_t nq, const float* __restrict__ R,
+                      uint64_t ldr, uint32_t nr, uint32_t dim, double* __restrict__ out, uint64_t ldo)
+{
+    const uint32_t q0 = blockIdx.y * kPairTile;
+    const uint32_t r0 = blockIdx.x * kPairTile;
+    double acc[4][4];
+    pair_tile_fp64<L1>(Q, ldq, nq, q0, R, ldr, nr, r0, dim, acc);
+    const int ti = threadIdx.x / 16;
+    const int tj = threadIdx.x % 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t q = q0 + ti * 4 + i;
+            const uint32_t r = r0 + tj * 4 + j;
+            if (q < nq && r < nr) out[static_cast<uint64_t>(q) * ldo + r] = L1 ? acc[i][j] : __dsqrt_rn(acc[i][j]);
         }
     }
 }
@@ -198,6 +239,20 @@ __global__ void zero_u64_kernel(unsigned long long* p, uint64_t n)
 }
 
 }  // namespace
+
+void distance_table(bool l1, const float* queries, uint64_t ldq, uint32_t nq, const float* refs, uint64_t ldr,
+                    uint32_t nr, uint32_t dim, double* out, uint64_t ldo, hipStream_t stream)
+{
+    if (nq == 0 || nr == 0) return;
+    const dim3 grid((nr + kPairTile - 1) / kPairTile, (nq + kPairTile - 1) / kPairTile);
+    if (grid.y > 65535U) throw std::length_error("distance table grid exceeds the launch limit");
+    if (l1) {
+        distance_table_kernel<true><<<grid, kPairThreads, 0, stream>>>(queries, ldq, nq, refs, ldr, nr, dim, out, ldo);
+    } else {
+        distance_table_kernel<false><<<grid, kPairThreads, 0, stream>>>(queries, ldq, nq, refs, ldr, nr, dim, out, ldo);
+    }
+    check_launch("distance table kernel");
+}
 
 void importance_probabilities(bool l1, const float* refs, uint64_t ld, uint32_t rows, uint32_t dim,
                               double* inv_scratch, unsigned long long* out_bits, hipStream_t stream)

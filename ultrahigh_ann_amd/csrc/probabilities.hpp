@@ -13,4 +13,11 @@ namespace uhann::kern {
 void importance_probabilities(bool l1, const float* refs, uint64_t ld, uint32_t rows, uint32_t dim,
                               double* inv_scratch, unsigned long long* out_bits, hipStream_t stream);
 
+// out[q][r] (row stride ldo) = the binary64 distance of query q and
+// reference r in the reference's CPU order: L2 sqrt(sum (q - r)^2) (the
+// diagnostic distance), L1 sum |q - r|. Queries nq x dim (stride ldq) and
+// references nr x dim (stride ldr) binary32 on the device.
+void distance_table(bool l1, const float* queries, uint64_t ldq, uint32_t nq, const float* refs, uint64_t ldr,
+                    uint32_t nr, uint32_t dim, double* out, uint64_t ldo, hipStream_t stream);
+
 }  // namespace uhann::kern
