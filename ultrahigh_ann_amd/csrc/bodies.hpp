@@ -146,42 +146,94 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
 
     const uint32_t col_t = static_cast<uint32_t>(RPT * t);
     const uint32_t full_end = (a.dim / kChains) * kChains;
-    if (a.stream_refs) {
-        // Single-pass streams keep more 256-column steps of loads in flight.
-        constexpr int kStreamUnroll = RPT == 4 && TQ * TR <= 8 ? 4 : 2;
-#pragma unroll kStreamUnroll
-        for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, true);
-    } else {
-#pragma unroll 2
-        for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, false);
-    }
-    if (full_end + col_t < a.dim) step(full_end + col_t, true, false);
-
-    // Replay the reference tree. Cross-thread steps first (chain offsets 16..RPT
-    // are thread offsets GROUP/2..1), then the chains inside the thread.
+    // Replay the reference tree for the chains of one pair (v: this thread's
+    // RPT chain sums): cross-thread steps first (chain offsets 16..RPT are
+    // thread offsets GROUP/2..1), then the chains inside the thread; lane 0 of
+    // each group of 32 chains stores the group sum for pair p.
     __shared__ float group_sums[kGroups][NP];
     __shared__ float pair_dist[NP];
+    auto pair_tree = [&](float (&v)[RPT], int p) {
 #pragma unroll
-    for (int i = 0; i < TQ; ++i) {
+        for (int off = GROUP / 2; off >= 1; off /= 2) {
 #pragma unroll
-        for (int j = 0; j < TR; ++j) {
-            float v[RPT];
+            for (int k = 0; k < RPT; ++k) v[k] = v[k] + __shfl_down(v[k], off);
+        }
+        float w;
+        if constexpr (RPT == 1) {
+            w = v[0];
+        } else if constexpr (RPT == 2) {
+            w = v[0] + v[1];
+        } else {
+            w = (v[0] + v[2]) + (v[1] + v[3]);
+        }
+        if (t % GROUP == 0) group_sums[t / GROUP][p] = w;
+    };
+
+    bool rows_done = false;
+    if constexpr (RPT == 4 && TQ == 1 && TR > 1) {
+        if (a.stream_refs) {
+            // One query streaming its references: the rows are read one after
+            // the other (one sequential stream per wave, which HBM serves
+            // better than TR interleaved ones), eight 256-column steps of
+            // loads in flight, and each row's tree runs before the next row.
+            // Each pair's chains see the same columns in the same order as in
+            // the interleaved form.
+#pragma unroll 1
+            for (int j = 0; j < TR; ++j) {
+                const float* rr = a.refs + static_cast<uint64_t>(min(r0 + j, a.rows - 1U)) * a.ld_ref;
+                float v[RPT];
 #pragma unroll
-            for (int k = 0; k < RPT; ++k) v[k] = acc[i][j][k];
+                for (int k = 0; k < RPT; ++k) v[k] = 0.0F;
+                auto step1 = [&](uint32_t col, bool masked) {
+                    float rv[RPT];
+                    float qv[RPT];
+                    float wv[RPT];
+                    if (masked) {
+                        load_vec<RPT>(rr + col, rv);
+                    } else {
+                        load_vec_stream<RPT>(rr + col, rv);
+                    }
+                    load_vec<RPT>(qrow[0] + col, qv);
+                    if constexpr (WEIGHTED) {
+                        load_vec<RPT>(a.weights + col, wv);
+                    } else {
 #pragma unroll
-            for (int off = GROUP / 2; off >= 1; off /= 2) {
+                        for (int k = 0; k < RPT; ++k) wv[k] = 0.0F;
+                    }
 #pragma unroll
-                for (int k = 0; k < RPT; ++k) v[k] = v[k] + __shfl_down(v[k], off);
+                    for (int k = 0; k < RPT; ++k) {
+                        if (masked && col + static_cast<uint32_t>(k) >= a.dim) continue;
+                        v[k] = accumulate<METRIC, WEIGHTED, WFMA>(v[k], rv[k], qv[k], wv[k]);
+                    }
+                };
+#pragma unroll 4
+                for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step1(c0 + col_t, false);
+                if (full_end + col_t < a.dim) step1(full_end + col_t, true);
+                pair_tree(v, j);
             }
-            float w;
-            if constexpr (RPT == 1) {
-                w = v[0];
-            } else if constexpr (RPT == 2) {
-                w = v[0] + v[1];
-            } else {
-                w = (v[0] + v[2]) + (v[1] + v[3]);
+            rows_done = true;
+        }
+    }
+    if (!rows_done) {
+        if (a.stream_refs) {
+            // Single-pass streams keep more 256-column steps of loads in flight.
+            constexpr int kStreamUnroll = RPT == 4 && TQ * TR <= 8 ? 4 : 2;
+#pragma unroll kStreamUnroll
+            for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, true);
+        } else {
+#pragma unroll 2
+            for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, false);
+        }
+        if (full_end + col_t < a.dim) step(full_end + col_t, true, false);
+#pragma unroll
+        for (int i = 0; i < TQ; ++i) {
+#pragma unroll
+            for (int j = 0; j < TR; ++j) {
+                float v[RPT];
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) v[k] = acc[i][j][k];
+                pair_tree(v, i * TR + j);
             }
-            if (t % GROUP == 0) group_sums[t / GROUP][i * TR + j] = w;
         }
     }
     __syncthreads();
