@@ -360,14 +360,20 @@ gram_bf16_kernel(const __bf16* __restrict__ Q, uint64_t ldq, uint32_t M,
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0F;
 
+    // The loads of step s + 1 are issued as soon as step s sits in LDS (before
+    // the barrier), so they have the barrier wait and the MFMAs of step s to
+    // arrive. The last iteration re-loads the last step (unused), which keeps
+    // the loads unconditional.
+    const uint32_t k_last = gb.k_end - kBK;
     UHANN_GRAM_LOAD(gb.k_begin, ra, rb)
     UHANN_GRAM_STORE(0, ra, rb)
+    UHANN_GRAM_LOAD(min(gb.k_begin + kBK, k_last), ra, rb)
     __syncthreads();
     int cur = 0;
     for (uint32_t k0 = gb.k_begin + kBK; k0 < gb.k_end; k0 += kBK) {
-        UHANN_GRAM_LOAD(k0, ra, rb)
         UHANN_GRAM_COMPUTE(cur)
         UHANN_GRAM_STORE(cur ^ 1, ra, rb)
+        UHANN_GRAM_LOAD(min(k0 + kBK, k_last), ra, rb)
         __syncthreads();
         cur ^= 1;
     }
@@ -810,14 +816,17 @@ gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const u
         acc[1][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a1, b1, acc[1][1], 0, 0, 0, 127, 0, 127); \
     }
 
+    // Loads of the next step issue before the barrier, as in the BF16 kernel.
+    const uint32_t k_last = gb.k_end - kBK8;
     UHANN_FP8_LOAD(gb.k_begin)
     UHANN_FP8_STORE(0)
+    UHANN_FP8_LOAD(min(gb.k_begin + kBK8, k_last))
     __syncthreads();
     int cur = 0;
     for (uint32_t k0 = gb.k_begin + kBK8; k0 < gb.k_end; k0 += kBK8) {
-        UHANN_FP8_LOAD(k0)
         UHANN_FP8_COMPUTE(cur)
         UHANN_FP8_STORE(cur ^ 1)
+        UHANN_FP8_LOAD(min(k0 + kBK8, k_last))
         __syncthreads();
         cur ^= 1;
     }
