@@ -198,6 +198,16 @@ def lib():
                 f"HIP engine library not built: {LIB_PATH} is missing "
                 "(run __graft_entry__.build() or make -C ultrahigh_ann_amd/csrc)"
             )
+        # torch carries its own HIP runtime. When both it and the engine's
+        # (/opt/rocm) live in one process, torch's must initialise first:
+        # started second it finds no device. Initialise it before loading
+        # the engine, whatever order the caller imports things in.
+        try:
+            import torch
+
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
 
