@@ -37,7 +37,7 @@ PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "hbm": 8000.0}
 # (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy kernel's readings
 # (profiles/exact_l2_r7028_d60660_b4096_pmc_traffic.md). Keyed by
 # (references, dim, batch); other shapes report null.
-GRAM_TRAFFIC_BYTES = {(7028, 60660, 4096): 5.718e9 + 0.156e9}
+GRAM_TRAFFIC_BYTES = {(7028, 60660, 4096): 5.220e9 + 0.291e9}
 
 
 def log(msg):
