@@ -417,8 +417,161 @@ __device__ __forceinline__ float unordered(uint32_t u)
 }
 
 constexpr int kSelThreads = 256;
+constexpr int kSelWaves = kSelThreads / 64;
 constexpr int kRerankRows = 4;
 constexpr int kKeepLo = 32;  // approx - tau values kept in registers per thread
+
+// Binary32 distances of one query row to R reference rows in the reference's
+// order, computed by ONE wave. The 256 chains (chain c adds columns c,
+// c + 256, ... in order with fmaf) sit four to a lane: lane l holds chains
+// 4l .. 4l + 3 and loads them as one 16-byte vector per 256-column stripe.
+// The reduction pairs the chains exactly as the 256-thread block tree does:
+// inside each 32-chain group offsets 16, 8, 4 (lanes l + 4, l + 2, l + 1),
+// then 2 and 1 (components of one lane), then the eight group sums
+// ((s0 + s4) + (s2 + s6)) + ((s1 + s5) + (s3 + s7)). Rows must be 16-byte
+// aligned; the last partial stripe is read element by element (nothing past
+// column dim - 1 is touched). The distances are returned in every lane.
+template <int R>
+__device__ __forceinline__ void wave_distances(const float* __restrict__ qrow, const float* const (&rrow)[R],
+                                               uint32_t dim, float (&out)[R])
+{
+    const int lane = threadIdx.x % 64;
+    float acc[R][4];
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[j][c] = 0.0F;
+    const uint32_t stripes = dim / 256U;
+    const float4* q4 = reinterpret_cast<const float4*>(qrow) + lane;
+    const float4* r4[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) r4[j] = reinterpret_cast<const float4*>(rrow[j]) + lane;
+    auto step = [&](const float4& qv, const float4& rv, float (&a)[4]) {
+        float d = rv.x - qv.x;
+        a[0] = __builtin_fmaf(d, d, a[0]);
+        d = rv.y - qv.y;
+        a[1] = __builtin_fmaf(d, d, a[1]);
+        d = rv.z - qv.z;
+        a[2] = __builtin_fmaf(d, d, a[2]);
+        d = rv.w - qv.w;
+        a[3] = __builtin_fmaf(d, d, a[3]);
+    };
+    constexpr int kU = 4;  // stripes loaded together
+    uint32_t s = 0;
+    for (; s + kU <= stripes; s += kU) {
+        float4 qv[kU], rv[kU][R];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            qv[u] = q4[(s + u) * 64U];
+#pragma unroll
+            for (int j = 0; j < R; ++j) rv[u][j] = r4[j][(s + u) * 64U];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+#pragma unroll
+            for (int j = 0; j < R; ++j) step(qv[u], rv[u][j], acc[j]);
+    }
+    for (; s < stripes; ++s) {
+        const float4 qv = q4[s * 64U];
+#pragma unroll
+        for (int j = 0; j < R; ++j) step(qv, r4[j][s * 64U], acc[j]);
+    }
+    const uint32_t c0 = stripes * 256U + 4U * static_cast<uint32_t>(lane);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (c0 + c < dim) {
+            const float qv = qrow[c0 + c];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const float d = rrow[j][c0 + c] - qv;
+                acc[j][c] = __builtin_fmaf(d, d, acc[j][c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float x = acc[j][c];
+            x = x + __shfl_down(x, 4);  // chain offset 16
+            x = x + __shfl_down(x, 2);  // chain offset 8
+            x = x + __shfl_down(x, 1);  // chain offset 4
+            v[c] = x;
+        }
+        const float group = (v[0] + v[2]) + (v[1] + v[3]);  // offsets 2, 1; valid in lanes 8G
+        float s8[8];
+#pragma unroll
+        for (int g = 0; g < 8; ++g) s8[g] = __shfl(group, 8 * g);
+        out[j] = ((s8[0] + s8[4]) + (s8[2] + s8[6])) + ((s8[1] + s8[5]) + (s8[3] + s8[7]));
+    }
+}
+
+// Binary64 squared distance of two binary32 rows computed by one wave: lane l
+// adds columns 256 s + 4 l + c (16-byte rows) or l + 64 s (any rows) in order,
+// then a butterfly over the wave; the value is returned in every lane. Used
+// only to order candidates inside the binary32 error band.
+template <bool VEC>
+__device__ __forceinline__ double wave_distance_f64(const float* __restrict__ qrow, const float* __restrict__ rrow,
+                                                    uint32_t dim)
+{
+    const uint32_t lane = threadIdx.x % 64U;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    auto sq = [](float r, float q) {
+        const double d = static_cast<double>(r) - static_cast<double>(q);
+        return d * d;
+    };
+    if constexpr (VEC) {
+        const uint32_t stripes = dim / 256U;
+        const float4* q4 = reinterpret_cast<const float4*>(qrow) + lane;
+        const float4* r4 = reinterpret_cast<const float4*>(rrow) + lane;
+        constexpr int kU = 4;
+        uint32_t s = 0;
+        for (; s + kU <= stripes; s += kU) {
+            float4 qv[kU], rv[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                qv[u] = q4[(s + u) * 64U];
+                rv[u] = r4[(s + u) * 64U];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                acc[0] += sq(rv[u].x, qv[u].x);
+                acc[1] += sq(rv[u].y, qv[u].y);
+                acc[2] += sq(rv[u].z, qv[u].z);
+                acc[3] += sq(rv[u].w, qv[u].w);
+            }
+        }
+        for (; s < stripes; ++s) {
+            const float4 qv = q4[s * 64U];
+            const float4 rv = r4[s * 64U];
+            acc[0] += sq(rv.x, qv.x);
+            acc[1] += sq(rv.y, qv.y);
+            acc[2] += sq(rv.z, qv.z);
+            acc[3] += sq(rv.w, qv.w);
+        }
+        const uint32_t c0 = stripes * 256U + 4U * lane;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (c0 + c < dim) acc[c] += sq(rrow[c0 + c], qrow[c0 + c]);
+    } else {
+        uint32_t c = lane;
+        for (; c + 3U * 64U < dim; c += 4U * 64U) {
+            float qv[4], rv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                qv[u] = qrow[c + u * 64U];
+                rv[u] = rrow[c + u * 64U];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += sq(rv[u], qv[u]);
+        }
+        for (; c < dim; c += 64U) acc[0] += sq(rrow[c], qrow[c]);
+    }
+    double v = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    for (int off = 32; off >= 1; off /= 2) v += __shfl_xor(v, off);
+    return v;
+}
 
 // One block per query. See the file comment for the algorithm.
 // Exact re-rank of `count` candidate rows s_cand of query q (ROWS per pass
@@ -428,14 +581,39 @@ constexpr int kKeepLo = 32;  // approx - tau values kept in registers per thread
 // the binary32 error band. Uses the whole block.
 template <int ROWS>
 __device__ void rerank_pick(const SelectArgs& a, uint32_t q, const uint32_t* s_cand, uint32_t count, float* s_dist,
-                            float (*s_group)[ROWS], double* s_scratch, uint32_t* s_best, float* s_best_d,
+                            float (*s_group)[ROWS], double* s_d64, uint32_t* s_best, float* s_best_d,
                             uint32_t& best, float& best_d)
 {
     constexpr int kRerankRows = ROWS;  // candidate rows re-ranked per pass over the query row
     const int t = threadIdx.x;
-    // Exact binary32 re-rank in the reference's order: thread t is chain t.
     const float* qrow = a.queries + static_cast<uint64_t>(q) * a.ld_q;
-    for (uint32_t g = 0; g < count; g += kRerankRows) {
+    // 16-byte aligned rows: each wave re-ranks its own candidates (ROWS /
+    // kSelWaves per pass over the query row), four chains per lane.
+    const bool vec = a.ld_q % 4U == 0 && a.ld_ref % 4U == 0 && (reinterpret_cast<uintptr_t>(a.queries) & 15U) == 0 &&
+                     (reinterpret_cast<uintptr_t>(a.refs) & 15U) == 0;
+    constexpr int kWaveRows = ROWS / kSelWaves > 0 ? ROWS / kSelWaves : 1;
+    if (vec) {
+        const uint32_t wave = static_cast<uint32_t>(t) / 64U;
+        for (uint32_t g = wave * kWaveRows; g < count; g += kSelWaves * kWaveRows) {
+            const float* rrow[kWaveRows];
+#pragma unroll
+            for (int j = 0; j < kWaveRows; ++j) {
+                const uint32_t r = s_cand[min(g + static_cast<uint32_t>(j), count - 1U)];
+                rrow[j] = a.refs + static_cast<uint64_t>(r) * a.ld_ref;
+            }
+            float d[kWaveRows];
+            wave_distances<kWaveRows>(qrow, rrow, a.dim, d);
+            if (t % 64 == 0) {
+#pragma unroll
+                for (int j = 0; j < kWaveRows; ++j)
+                    if (g + j < count) s_dist[g + j] = d[j];
+            }
+        }
+        __syncthreads();
+    }
+    // Otherwise the exact binary32 re-rank in the reference's order with
+    // thread t as chain t.
+    for (uint32_t g = 0; !vec && g < count; g += kRerankRows) {
         const uint32_t ng = min(static_cast<uint32_t>(kRerankRows), count - g);
         const float* rrow[kRerankRows];
 #pragma unroll
@@ -507,24 +685,30 @@ __device__ void rerank_pick(const SelectArgs& a, uint32_t q, const uint32_t* s_c
     best = *s_best;
     best_d = *s_best_d;
 
-    // Binary64 re-rank of candidates inside the binary32 error band.
+    // Binary64 re-rank of candidates inside the binary32 error band: the
+    // waves take the band's candidates in turn, then the binary64 minimum
+    // (lowest row on ties) wins.
     const double band = static_cast<double>(best_d) * a.fp32_band;
     uint32_t near = 0;
     for (uint32_t i = 0; i < count; ++i) near += (static_cast<double>(s_dist[i]) <= band) ? 1U : 0U;
     if (near >= 2) {
+        const uint32_t wave = static_cast<uint32_t>(t) / 64U;
+        uint32_t k = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            if (static_cast<double>(s_dist[i]) > band) continue;
+            if (k++ % kSelWaves != wave) continue;
+            const float* rr = a.refs + static_cast<uint64_t>(s_cand[i]) * a.ld_ref;
+            const double d64 = vec ? wave_distance_f64<true>(qrow, rr, a.dim) : wave_distance_f64<false>(qrow, rr, a.dim);
+            if (t % 64 == 0) s_d64[i] = d64;
+        }
+        __syncthreads();
         double best64 = __longlong_as_double(0x7ff0000000000000LL);
         uint32_t best64_r = 0xffffffffU;
         float best64_f = 0.0F;
         for (uint32_t i = 0; i < count; ++i) {
             if (static_cast<double>(s_dist[i]) > band) continue;
             const uint32_t r = s_cand[i];
-            const float* rr = a.refs + static_cast<uint64_t>(r) * a.ld_ref;
-            double part = 0.0;
-            for (uint32_t c = t; c < a.dim; c += kSelThreads) {
-                const double d = static_cast<double>(rr[c]) - static_cast<double>(qrow[c]);
-                part += d * d;
-            }
-            const double d64 = block_sum_f64(part, s_scratch);
+            const double d64 = s_d64[i];
             if (d64 < best64 || (d64 == best64 && r < best64_r)) {
                 best64 = d64;
                 best64_r = r;
@@ -545,7 +729,7 @@ select_rerank_kernel(SelectArgs a)
     __shared__ uint32_t s_cand[kMaxCandidates];
     __shared__ float s_dist[kMaxCandidates];
     __shared__ float s_group[8][kRerankRows];
-    __shared__ double s_scratch[8];
+    __shared__ double s_d64[kMaxCandidates];
     __shared__ uint32_t s_best;
     __shared__ float s_best_d;
 
@@ -614,7 +798,7 @@ select_rerank_kernel(SelectArgs a)
 
     uint32_t best;
     float best_d;
-    rerank_pick<kRerankRows>(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
+    rerank_pick<kRerankRows>(a, q, s_cand, count, s_dist, s_group, s_d64, &s_best, &s_best_d, best, best_d);
     if (t == 0) {
         a.out_idx[q] = best;
         a.out_dist[q] = best_d;
@@ -851,13 +1035,14 @@ __global__ void __launch_bounds__(kSelThreads)
 select_topk_rerank_kernel(SelectArgs a)
 {
     __shared__ unsigned int s_hist[256];
+    __shared__ uint32_t s_wsum[kSelWaves];
     __shared__ unsigned long long s_prefix;
     __shared__ uint32_t s_need;
     __shared__ uint32_t s_count;
     __shared__ uint32_t s_cand[kShortlist];
     __shared__ float s_dist[kShortlist];
     __shared__ float s_group[8][kShortlist];
-    __shared__ double s_scratch[8];
+    __shared__ double s_d64[kShortlist];
     __shared__ uint32_t s_best;
     __shared__ float s_best_d;
     __shared__ int s_uncertified;
@@ -891,12 +1076,18 @@ select_topk_rerank_kernel(SelectArgs a)
         s_count = 0;
         s_uncertified = 0;
     }
+    // Radix select, 8 bits per pass from the top: the bin holding the
+    // need-th smallest remaining key is found by a block-wide prefix sum of
+    // the histogram (thread t holds bin t).
+    const int lane = t % 64;
+    const int wave = t / 64;
     unsigned long long mask = 0ULL;
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 56 - 8 * pass;
         s_hist[t] = 0U;
         __syncthreads();
         const unsigned long long prefix = s_prefix;
+        const uint32_t need = s_need;
 #pragma unroll
         for (int i = 0; i < kShortlist; ++i) {
             if (top[i] != ~0ULL && (top[i] & mask) == prefix) {
@@ -904,15 +1095,19 @@ select_topk_rerank_kernel(SelectArgs a)
             }
         }
         __syncthreads();
-        if (t == 0) {
-            uint32_t need = s_need;
-            uint32_t bin = 0;
-            for (; bin < 255U; ++bin) {
-                if (s_hist[bin] >= need) break;
-                need -= s_hist[bin];
-            }
-            s_need = need;
-            s_prefix = prefix | (static_cast<unsigned long long>(bin) << shift);
+        const uint32_t h = s_hist[t];
+        uint32_t incl = h;
+        for (int off = 1; off < 64; off *= 2) {
+            const uint32_t v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) incl += s_wsum[w];
+        const uint32_t excl = incl - h;
+        if (excl < need && need <= incl) {  // exactly one bin when 0 < need <= total
+            s_need = need - excl;
+            s_prefix = prefix | (static_cast<unsigned long long>(t) << shift);
         }
         mask |= 255ULL << shift;
         __syncthreads();
@@ -930,7 +1125,7 @@ select_topk_rerank_kernel(SelectArgs a)
 
     uint32_t best;
     float best_d;
-    rerank_pick<kShortlist>(a, q, s_cand, count, s_dist, s_group, s_scratch, &s_best, &s_best_d, best, best_d);
+    rerank_pick<kShortlist>(a, q, s_cand, count, s_dist, s_group, s_d64, &s_best, &s_best_d, best, best_d);
 
     // Certified when no row outside the shortlist can be closer: its
     // estimate minus its error bound exceeds the answer's distance (widened
