@@ -104,15 +104,26 @@ to_bf16_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t row
     uint32_t c0 = 0;
     if (vec) {
         const uint32_t full = dim / 4 * 4;
-        for (uint32_t c = threadIdx.x * 4; c < full; c += blockDim.x * 4) {
-            const float4 x = *reinterpret_cast<const float4*>(s + c);
+        const uint32_t step = blockDim.x * 4;
+        auto quad = [&](const float4& x, uint32_t c) {
             bf16x4 h;
             h[0] = one(x.x);
             h[1] = one(x.y);
             h[2] = one(x.z);
             h[3] = one(x.w);
             *reinterpret_cast<bf16x4*>(o + c) = h;
+        };
+        // Four loads in flight per thread; each thread still visits its
+        // columns in increasing order.
+        uint32_t c = threadIdx.x * 4;
+        for (; c + 3 * step < full; c += 4 * step) {
+            float4 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const float4*>(s + c + u * step);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) quad(x[u], c + u * step);
         }
+        for (; c < full; c += step) quad(*reinterpret_cast<const float4*>(s + c), c);
         c0 = full;
     }
     for (uint32_t c = c0 + threadIdx.x; c < ld_dst; c += blockDim.x) o[c] = one(c < dim ? s[c] : 0.0F);
@@ -835,7 +846,7 @@ __device__ __forceinline__ float e4m3_value(uint32_t b)
     const uint32_t e = (b >> 3) & 0xfU;
     const uint32_t m = b & 7U;
     const float mag = e == 0 ? static_cast<float>(m) * 0.001953125F  // m * 2^-9
-                             : ldexpf(1.0F + static_cast<float>(m) * 0.125F, static_cast<int>(e) - 7);
+                             : __uint_as_float(((e + 120U) << 23) | (m << 20));  // (1 + m/8) 2^(e-7)
     return (b & 0x80U) ? -mag : mag;
 }
 
@@ -867,9 +878,17 @@ to_fp8_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows
     const bool vec = ld_src % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15U) == 0;
     const uint32_t full = vec ? dim / 4 * 4 : 0;
     float mx = 0.0F;
-    for (uint32_t c = threadIdx.x * 4; c < full; c += blockDim.x * 4) {
-        const float4 x = *reinterpret_cast<const float4*>(s + c);
-        mx = fmaxf(fmaxf(mx, fmaxf(fabsf(x.x), fabsf(x.y))), fmaxf(fabsf(x.z), fabsf(x.w)));
+    {
+        auto qmax = [](const float4& x) { return fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))); };
+        const uint32_t step = blockDim.x * 4;
+        uint32_t c = threadIdx.x * 4;
+        for (; c + 3 * step < full; c += 4 * step) {
+            float4 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const float4*>(s + c + u * step);
+            mx = fmaxf(mx, fmaxf(fmaxf(qmax(x[0]), qmax(x[1])), fmaxf(qmax(x[2]), qmax(x[3]))));
+        }
+        for (; c < full; c += step) mx = fmaxf(mx, qmax(*reinterpret_cast<const float4*>(s + c)));
     }
     for (uint32_t c = full + threadIdx.x; c < dim; c += blockDim.x) mx = fmaxf(mx, fabsf(s[c]));
     mx = block_max_f32(mx, fscratch);
@@ -882,9 +901,12 @@ to_fp8_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows
         c_exp = c_exp < -126 ? -126 : c_exp;
     }
     const float scale = ldexpf(1.0F, c_exp);
+    // 2^-c_exp is a normal binary32 (c_exp in [-126, 120]), so x * inv is
+    // the correctly rounded x 2^-c_exp, as ldexpf gives.
+    const float inv = ldexpf(1.0F, -c_exp);
     double n2 = 0.0, n4 = 0.0, h2 = 0.0, h4 = 0.0, e4 = 0.0;
     auto one = [&](float x) {
-        const uint32_t b = e4m3_rne(ldexpf(x, -c_exp));
+        const uint32_t b = e4m3_rne(x * inv);
         const double xd = x;
         const double hd = static_cast<double>(e4m3_value(b)) * static_cast<double>(scale);
         const double ed = xd - hd;
@@ -896,11 +918,20 @@ to_fp8_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows
         return b;
     };
     // ld_dst is a multiple of 128, so four-byte groups never straddle a row.
-    for (uint32_t c = threadIdx.x * 4; c < full; c += blockDim.x * 4) {
-        const float4 x = *reinterpret_cast<const float4*>(s + c);
+    const uint32_t step = blockDim.x * 4;
+    auto quad = [&](const float4& x, uint32_t c) {
         const uint32_t packed = one(x.x) | (one(x.y) << 8) | (one(x.z) << 16) | (one(x.w) << 24);
         *reinterpret_cast<uint32_t*>(o + c) = packed;
+    };
+    uint32_t c4 = threadIdx.x * 4;
+    for (; c4 + 3 * step < full; c4 += 4 * step) {
+        float4 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const float4*>(s + c4 + u * step);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) quad(x[u], c4 + u * step);
     }
+    for (; c4 < full; c4 += step) quad(*reinterpret_cast<const float4*>(s + c4), c4);
     for (uint32_t c = full + threadIdx.x; c < ld_dst; c += blockDim.x) {
         o[c] = static_cast<uint8_t>(one(c < dim ? s[c] : 0.0F));
     }
