@@ -144,6 +144,34 @@ def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
     np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
 
 
+STREAM_TILES = [t for t in TILES if t[0] == 4]
+
+
+@pytest.mark.parametrize("tile", STREAM_TILES)
+@pytest.mark.parametrize("d", [1000, 516, 4096])
+def test_stream_tiles_bit_exact(uh, oracle_mod, tile, d):
+    """A batch that fits one query tile streams its references (rows read one
+    after the other per wave): same bits as the oracle, also for a partial
+    query tile and with an L1 weighted index."""
+    import torch
+
+    r = 45
+    for b in sorted({tile[1], max(1, tile[1] - 1)}):
+        refs, q = _data(11 + d + b, r, d, b)
+        dq = torch.from_numpy(q).cuda()
+        oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+        od = torch.zeros(b, dtype=torch.float32, device="cuda")
+        w = (rng(d + 1).random(d, dtype=np.float32) * 3 + 0.1).astype(F32)
+        for index, expect in ((uh.HipDenseL2ScanIndex(refs), oracle_mod.k2_distances(refs, q)),
+                              (uh.HipDenseL1ScanIndex(refs, w), oracle_mod.k5_distances(refs, q, w, 0))):
+            ws = index.make_query_workspace(b)
+            exp_idx, exp_dist = oracle_mod.fp32_argmin(expect)
+            index.query_device_tile(dq, b, ws, oi, od, tile, stream=torch.cuda.current_stream())
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+            np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+
+
 def test_device_batch_and_async(uh, oracle_mod):
     import torch
 
