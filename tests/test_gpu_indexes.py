@@ -263,7 +263,8 @@ def test_hierarchical_l1_reference_fixture(uh, oracle_mod):
 
 
 @pytest.mark.parametrize("metric_l1", [False, True])
-@pytest.mark.parametrize("r,d,b,T,p", [(80, 1200, 21, 32, 16), (250, 3000, 33, 32, 64), (40, 700, 9, 8, 6)])
+@pytest.mark.parametrize("r,d,b,T,p", [(80, 1200, 21, 32, 16), (250, 3000, 33, 32, 64), (40, 700, 9, 8, 6),
+                                       (60, 900, 11, 16, 32), (50, 800, 7, 16, 128), (30, 500, 5, 8, 8)])
 def test_hierarchical_parity_bit_exact(uh, oracle_mod, metric_l1, r, d, b, T, p):
     refs, q = _random_index_case(r * 7 + p, r, d, b)
     probs = oracle_mod.importance_probabilities(metric_l1, refs)

@@ -550,7 +550,8 @@ __device__ __forceinline__ void bitonic_sort(float (&v)[N])
 }
 
 // Median-L1 estimates of query q against references rblock*256 ... +255.
-template <int PMAX>
+// EXACT: p == PMAX, so the middle positions are compile-time constants.
+template <int PMAX, bool EXACT>
 __device__ __forceinline__ void median_block(const float* __restrict__ refs_t, uint32_t rows,
                                              const float* __restrict__ yq, uint64_t ld_y, uint32_t nq, uint32_t p,
                                              unsigned long long* __restrict__ keys, int* invalid, uint32_t q,
@@ -581,15 +582,22 @@ __device__ __forceinline__ void median_block(const float* __restrict__ refs_t, u
             }
         }
         bitonic_sort<PMAX>(v);
-        const uint32_t mid = p / 2;
         float med = 0.0F;
-        float lower = 0.0F;
+        if constexpr (EXACT) {
+            // Fixed middle positions: the compiler keeps only the
+            // compare-exchanges that feed them (in the last merge stage 2, 4,
+            // ..., then all of the two widest levels).
+            med = v[PMAX / 2 - 1] + (v[PMAX / 2] - v[PMAX / 2 - 1]) * 0.5F;
+        } else {
+            const uint32_t mid = p / 2;
+            float lower = 0.0F;
 #pragma unroll
-        for (int c = 0; c < PMAX; ++c) {
-            if (static_cast<uint32_t>(c) == mid) med = v[c];
-            if (static_cast<uint32_t>(c) + 1U == mid) lower = v[c];
+            for (int c = 0; c < PMAX; ++c) {
+                if (static_cast<uint32_t>(c) == mid) med = v[c];
+                if (static_cast<uint32_t>(c) + 1U == mid) lower = v[c];
+            }
+            if (p % 2 == 0) med = lower + (med - lower) * 0.5F;
         }
-        if (p % 2 == 0) med = lower + (med - lower) * 0.5F;
         if (!__builtin_isfinite(med)) bad = true;
         if (bad) atomicExch(invalid, 1);
         key = (static_cast<unsigned long long>(__float_as_uint(med)) << 32) | r;

@@ -73,13 +73,13 @@ project_bf16_kernel(const float* __restrict__ x, uint64_t ld_x, uint32_t nq, uin
     body::project_bf16_tile(x, ld_x, nq, k, proj, ld_proj, p, y, ld_y, invalid, blockIdx.x * 32, blockIdx.y * 32);
 }
 
-template <int PMAX>
+template <int PMAX, bool EXACT>
 __global__ void __launch_bounds__(kMedianThreads)
 median_l1_kernel(const float* __restrict__ refs_t, uint32_t rows, const float* __restrict__ yq,
                  uint64_t ld_y, uint32_t nq, uint32_t p, unsigned long long* __restrict__ keys,
                  int* invalid)
 {
-    body::median_block<PMAX>(refs_t, rows, yq, ld_y, nq, p, keys, invalid, blockIdx.y, blockIdx.x);
+    body::median_block<PMAX, EXACT>(refs_t, rows, yq, ld_y, nq, p, keys, invalid, blockIdx.y, blockIdx.x);
 }
 
 __global__ void zero_results_kernel(uint64_t* idx, float* dist, uint32_t n)
@@ -133,17 +133,25 @@ void median_l1(const float* refs_t, uint32_t rows, const float* yq, uint64_t ld_
         const dim3 grid((rows + kMedianThreads - 1) / kMedianThreads, n);
         const float* yb = yq + static_cast<uint64_t>(q0) * ld_y;
         unsigned long long* kb = keys + q0;
+        // p equal to the network size: the kernel with fixed middle positions.
+#define UHANN_MEDIAN(PM)                                                                                   \
+    if (p == PM) {                                                                                         \
+        median_l1_kernel<PM, true><<<grid, kMedianThreads, 0, stream>>>(refs_t, rows, yb, ld_y, n, p, kb, invalid); \
+    } else {                                                                                               \
+        median_l1_kernel<PM, false><<<grid, kMedianThreads, 0, stream>>>(refs_t, rows, yb, ld_y, n, p, kb, invalid); \
+    }
         if (p <= 8) {
-            median_l1_kernel<8><<<grid, kMedianThreads, 0, stream>>>(refs_t, rows, yb, ld_y, n, p, kb, invalid);
+            UHANN_MEDIAN(8)
         } else if (p <= 16) {
-            median_l1_kernel<16><<<grid, kMedianThreads, 0, stream>>>(refs_t, rows, yb, ld_y, n, p, kb, invalid);
+            UHANN_MEDIAN(16)
         } else if (p <= 32) {
-            median_l1_kernel<32><<<grid, kMedianThreads, 0, stream>>>(refs_t, rows, yb, ld_y, n, p, kb, invalid);
+            UHANN_MEDIAN(32)
         } else if (p <= 64) {
-            median_l1_kernel<64><<<grid, kMedianThreads, 0, stream>>>(refs_t, rows, yb, ld_y, n, p, kb, invalid);
+            UHANN_MEDIAN(64)
         } else {
-            median_l1_kernel<128><<<grid, kMedianThreads, 0, stream>>>(refs_t, rows, yb, ld_y, n, p, kb, invalid);
+            UHANN_MEDIAN(128)
         }
+#undef UHANN_MEDIAN
         check_launch("median-L1 kernel");
     }
 }

@@ -177,8 +177,8 @@ serve_median_kernel(const void* blob, int stage, unsigned long long* keys)
         const Unit w = locate(blob, stage, u);
         const ServeItem& it = *w.item;
         __syncthreads();
-        body::median_block<PMAX>(it.refs_t, it.rows, it.projected, it.ld_p, it.n, it.p, keys + it.qoff,
-                                 it.flags + 2, w.b, w.a);
+        body::median_block<PMAX, false>(it.refs_t, it.rows, it.projected, it.ld_p, it.n, it.p, keys + it.qoff,
+                                        it.flags + 2, w.b, w.a);
     }
 }
 
