@@ -148,7 +148,7 @@ STREAM_TILES = [t for t in TILES if t[0] == 4]
 
 
 @pytest.mark.parametrize("tile", STREAM_TILES)
-@pytest.mark.parametrize("d", [1000, 516, 4096])
+@pytest.mark.parametrize("d", [1000, 516, 4096, 3000])
 def test_stream_tiles_bit_exact(uh, oracle_mod, tile, d):
     """A batch that fits one query tile streams its references (rows read one
     after the other per wave): same bits as the oracle, also for a partial
