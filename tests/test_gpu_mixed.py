@@ -104,6 +104,47 @@ def test_mixed_escalation_to_direct_scan(uh, oracle_mod):
     np.testing.assert_array_equal(dist, exp_dist)
 
 
+def test_mixed_escalation_mixed_batch(uh, oracle_mod):
+    """Escalated and ordinary queries in one batch: 70 identical rows put more
+    than 64 candidates in the first 20 queries (several 8-query tiles of the
+    device-side direct scan), the other queries sit near distinct rows. Every
+    answer equals the oracle's binary32 argmin; the device path (strided
+    queries) agrees; a second batch without escalations is not affected by
+    the first one's list."""
+    import torch
+
+    g = rng(9)
+    d = 700
+    base = (g.random(d, dtype=np.float32) * 2).astype(F32)
+    refs = np.concatenate([np.tile(base, (70, 1)), (g.random((40, d), dtype=np.float32) * 2 + 3).astype(F32)])
+    near_base = base + (g.random((20, d), dtype=np.float32) * 0.01).astype(F32)
+    near_rows = refs[70 + g.integers(0, 40, 17)] + (g.random((17, d), dtype=np.float32) * 0.01).astype(F32)
+    q = np.concatenate([near_base, near_rows]).astype(F32)
+    b = q.shape[0]
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed()
+    ws = index.make_query_workspace(b)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q))
+    idx, dist = index.query_batch(q, b, ws, uh.STRATEGY_GRAM_BF16)
+    assert ws.mixed_stats()["escalated"] == 20
+    np.testing.assert_array_equal(idx, exp_idx)
+    np.testing.assert_array_equal(dist.view(np.uint32), exp_dist.view(np.uint32))
+    padded = torch.zeros((b, d + 3), dtype=torch.float32, device="cuda")
+    padded[:, :d] = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    index.query_device_async(padded, b, ws, oi, od, strategy=uh.STRATEGY_GRAM_BF16,
+                             stream=torch.cuda.current_stream(), ld_queries=d + 3)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+    tail = q[20:]
+    t_idx, t_dist = index.query_batch(tail, tail.shape[0], ws, uh.STRATEGY_GRAM_BF16)
+    assert ws.mixed_stats()["escalated"] == 0
+    np.testing.assert_array_equal(t_idx, exp_idx[20:])
+    np.testing.assert_array_equal(t_dist.view(np.uint32), exp_dist[20:].view(np.uint32))
+
+
 def test_mixed_requires_enable_and_l2(uh):
     refs = np.ones((3, 4), F32)
     index = uh.HipExactL2Index(refs)

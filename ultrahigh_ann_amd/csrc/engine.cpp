@@ -456,10 +456,7 @@ void Workspace::reserve_common(const Scan& scan)
     counters = DeviceBuffer<unsigned int>(device, 8);
     escalate_list = DeviceBuffer<unsigned int>(device, max_batch);
     esc_queries = DeviceBuffer<float>(device, checked_mul(max_batch, scan.ld, "escalation workspace"));
-    esc_idx = DeviceBuffer<uint64_t>(device, max_batch);
-    esc_dist = DeviceBuffer<float>(device, max_batch);
-    payload_bytes += approx.bytes() + counters.bytes() + escalate_list.bytes() + esc_queries.bytes() +
-                     esc_idx.bytes() + esc_dist.bytes();
+    payload_bytes += approx.bytes() + counters.bytes() + escalate_list.bytes() + esc_queries.bytes();
 }
 
 void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q,
@@ -510,19 +507,19 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
     a.stats = reinterpret_cast<kern::MixedStats*>(ws.counters.data() + 4);
     kern::select_rerank(a, stream);
     ws.mark(3, stream);
-    unsigned int escalated = 0;
-    check_hip(hipMemcpyAsync(&escalated, ws.counters.data(), sizeof(escalated), hipMemcpyDeviceToHost,
-                             stream),
-              "cannot read the escalation count");
-    check_hip(hipStreamSynchronize(stream), "mixed-precision query failed");
-    if (escalated > 0) {
-        kern::gather_rows(queries, ld_q, ws.escalate_list.data(), escalated,
-                          static_cast<uint32_t>(scan.dim), ws.esc_queries.data(), scan.ld, stream);
-        scan_launch(scan, ws.esc_queries.data(), scan.ld, escalated, ws.keys.data(), ws.esc_idx.data(),
-                    ws.esc_dist.data(), stream, true, nullptr);
-        kern::scatter_results(ws.esc_idx.data(), ws.esc_dist.data(), ws.escalate_list.data(), escalated,
-                              out_idx, out_dist, stream);
-    }
+    // Escalated queries get the exact direct scan on the device (the count
+    // stays in device memory; nothing waits for the host).
+    kern::ScanArgs esc;
+    esc.refs = scan.refs.data();
+    esc.ld_ref = scan.ld;
+    esc.rows = static_cast<uint32_t>(scan.rows);
+    esc.queries = queries;
+    esc.ld_q = ld_q;
+    esc.nq = n;
+    esc.dim = static_cast<uint32_t>(scan.dim);
+    esc.keys = ws.keys.data();
+    kern::escalate_scan(esc, ws.counters.data(), ws.escalate_list.data(), n, ws.esc_queries.data(), scan.ld,
+                        out_idx, out_dist, stream);
     ws.mark(4, stream);
     if (ws.timing) {
         check_hip(hipEventSynchronize(ws.events[4]), "HIP event wait failed");

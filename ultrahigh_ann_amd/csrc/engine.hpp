@@ -98,9 +98,7 @@ struct Workspace {
     DeviceBuffer<float> approx;
     DeviceBuffer<unsigned int> counters;  // [0] escalations, [1..4] MixedStats
     DeviceBuffer<unsigned int> escalate_list;
-    DeviceBuffer<float> esc_queries;
-    DeviceBuffer<uint64_t> esc_idx;
-    DeviceBuffer<float> esc_dist;
+    DeviceBuffer<float> esc_queries;    // escalated queries, gathered on the device
     DeviceBuffer<uint8_t> q8;           // FP8 strategy: e4m3 queries
     DeviceBuffer<kern::RowStats> q8stats;
     DeviceBuffer<float> gram_partials;  // K-split partial sums of the Gram tail tiles

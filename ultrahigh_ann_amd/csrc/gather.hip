@@ -51,27 +51,6 @@ __global__ void zero_u32_kernel(unsigned int* p, uint32_t n)
     if (i < n) p[i] = 0U;
 }
 
-// dst row i <- src row rows[i] (dim floats), one block per row.
-__global__ void gather_rows_kernel(const float* __restrict__ src, uint64_t ld_src,
-                                   const unsigned int* __restrict__ rows, uint32_t n, uint32_t dim,
-                                   float* __restrict__ dst, uint64_t ld_dst)
-{
-    const uint32_t i = blockIdx.x;
-    if (i >= n) return;
-    const float* s = src + static_cast<uint64_t>(rows[i]) * ld_src;
-    float* d = dst + static_cast<uint64_t>(i) * ld_dst;
-    for (uint32_t c = threadIdx.x; c < dim; c += blockDim.x) d[c] = s[c];
-}
-
-__global__ void scatter_results_kernel(const uint64_t* __restrict__ idx, const float* __restrict__ dist,
-                                       const unsigned int* __restrict__ rows, uint32_t n,
-                                       uint64_t* __restrict__ out_idx, float* __restrict__ out_dist)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (out_idx != nullptr) out_idx[rows[i]] = idx[i];
-    if (out_dist != nullptr) out_dist[rows[i]] = dist[i];
-}
 
 void check_launch(const char* what)
 {
@@ -110,22 +89,6 @@ void zero_u32(unsigned int* p, uint32_t n, hipStream_t stream)
     if (n == 0) return;
     zero_u32_kernel<<<(n + 255) / 256, 256, 0, stream>>>(p, n);
     check_launch("zero fill kernel");
-}
-
-void gather_rows(const float* src, uint64_t ld_src, const unsigned int* rows, uint32_t n, uint32_t dim,
-                 float* dst, uint64_t ld_dst, hipStream_t stream)
-{
-    if (n == 0) return;
-    gather_rows_kernel<<<n, 256, 0, stream>>>(src, ld_src, rows, n, dim, dst, ld_dst);
-    check_launch("row gather kernel");
-}
-
-void scatter_results(const uint64_t* idx, const float* dist, const unsigned int* rows, uint32_t n,
-                     uint64_t* out_idx, float* out_dist, hipStream_t stream)
-{
-    if (n == 0) return;
-    scatter_results_kernel<<<(n + 255) / 256, 256, 0, stream>>>(idx, dist, rows, n, out_idx, out_dist);
-    check_launch("result scatter kernel");
 }
 
 }  // namespace uhann::kern

@@ -108,11 +108,7 @@ uint32_t gram_k_padding();
 int gram_tile_count();
 uint32_t gram_m_tile();
 
-// Small helpers for the escalation path (scan.hip / gather.hip).
+// Zero fill of the strategy counters (gather.hip).
 void zero_u32(unsigned int* p, uint32_t n, hipStream_t stream);
-void gather_rows(const float* src, uint64_t ld_src, const unsigned int* rows, uint32_t n, uint32_t dim,
-                 float* dst, uint64_t ld_dst, hipStream_t stream);
-void scatter_results(const uint64_t* idx, const float* dist, const unsigned int* rows, uint32_t n,
-                     uint64_t* out_idx, float* out_dist, hipStream_t stream);
 
 }  // namespace uhann::kern

@@ -66,6 +66,13 @@ void scan_direct(Metric metric, bool weight_fma, const ScanArgs& args,
                  const ScanTile& tile, hipStream_t stream);
 void finalize_keys(const unsigned long long* keys, uint32_t n, uint64_t* out_idx,
                    float* out_dist, hipStream_t stream);
+// Exact binary32 L2 scan of the queries listed in list[0 .. min(*count, cap))
+// (indices into base.queries, row stride base.ld_q), with the count read on
+// the device: the listed rows are copied to staged (cap x ld_staged floats,
+// ld_staged >= base.dim), base.keys (cap entries) receives their keys, and
+// out_idx / out_dist[list[i]] the results. No host synchronisation.
+void escalate_scan(const ScanArgs& base, const unsigned int* count, const unsigned int* list, uint32_t cap,
+                   float* staged, uint64_t ld_staged, uint64_t* out_idx, float* out_dist, hipStream_t stream);
 
 // Pair-per-thread direct scan for dim <= scan_pairs_max_dim(): refs_t is the
 // transposed (dim x rows, row stride ld_t) reference matrix; tq (1, 2, 4)

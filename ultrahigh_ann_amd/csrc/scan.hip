@@ -81,6 +81,57 @@ scan_pairs_kernel(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows
                                                  blockIdx.y * TQ, blockIdx.x, smem);
 }
 
+// Device-side escalation of the mixed-precision strategy (no host round
+// trip): the queries its selection listed (list[0 .. *count)) are copied into
+// a contiguous block (their keys reset on the way), scanned exactly by a
+// fixed grid that walks (8-query tile, 8-row tile) work items with the
+// direct scan's tile body, and the results are written back to the listed
+// positions. With nothing listed every block returns at once.
+constexpr int kEscTQ = 8;
+constexpr int kEscTR = 8;
+constexpr uint32_t kEscBlocks = 1024;
+
+__global__ void __launch_bounds__(256)
+escalate_gather_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t dim,
+                       const unsigned int* __restrict__ count, const unsigned int* __restrict__ list, uint32_t cap,
+                       float* __restrict__ dst, uint64_t ld_dst, unsigned long long* __restrict__ keys)
+{
+    const uint32_t n = min(*count, cap);
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const float* s = src + static_cast<uint64_t>(list[i]) * ld_src;
+        float* d = dst + static_cast<uint64_t>(i) * ld_dst;
+        for (uint32_t c = threadIdx.x; c < ld_dst; c += blockDim.x) d[c] = c < dim ? s[c] : 0.0F;
+        if (threadIdx.x == 0) keys[i] = ~0ULL;
+    }
+}
+
+__global__ void __launch_bounds__(kChains)
+escalate_scan_kernel(ScanArgs a, const unsigned int* __restrict__ count, uint32_t cap)
+{
+    a.nq = min(*count, cap);
+    const uint32_t tiles_r = (a.rows + kEscTR - 1) / kEscTR;
+    const uint64_t work = static_cast<uint64_t>((a.nq + kEscTQ - 1) / kEscTQ) * tiles_r;
+    for (uint64_t w = blockIdx.x; w < work; w += gridDim.x) {
+        __syncthreads();
+        body::scan_tile<0, false, false, 1, kEscTQ, kEscTR>(a, static_cast<uint32_t>(w / tiles_r) * kEscTQ,
+                                                            static_cast<uint32_t>(w % tiles_r) * kEscTR);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+escalate_finalize_kernel(const unsigned long long* __restrict__ keys, const unsigned int* __restrict__ count,
+                         const unsigned int* __restrict__ list, uint32_t cap, uint64_t* __restrict__ out_idx,
+                         float* __restrict__ out_dist)
+{
+    const uint32_t n = min(*count, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned long long key = keys[i];
+        const uint32_t q = list[i];
+        out_idx[q] = key & 0xffffffffULL;
+        out_dist[q] = __uint_as_float(static_cast<uint32_t>(key >> 32));
+    }
+}
+
 __global__ void fill_keys_kernel(unsigned long long* keys, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -195,6 +246,23 @@ void scan_direct(Metric metric, bool weight_fma, const ScanArgs& args, const Sca
     hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kChains / tile.rpt), 0,
                        stream, args, tiles_q, tiles_r, group_q);
     check_launch("direct scan kernel");
+}
+
+void escalate_scan(const ScanArgs& base, const unsigned int* count, const unsigned int* list, uint32_t cap,
+                   float* staged, uint64_t ld_staged, uint64_t* out_idx, float* out_dist, hipStream_t stream)
+{
+    if (cap == 0 || base.rows == 0) return;
+    escalate_gather_kernel<<<min(cap, kEscBlocks), 256, 0, stream>>>(base.queries, base.ld_q, base.dim, count, list,
+                                                                     cap, staged, ld_staged, base.keys);
+    ScanArgs a = base;
+    a.queries = staged;
+    a.ld_q = ld_staged;
+    a.weights = nullptr;
+    a.stream_refs = false;
+    escalate_scan_kernel<<<kEscBlocks, kChains, 0, stream>>>(a, count, cap);
+    escalate_finalize_kernel<<<(cap + 255U) / 256U, 256, 0, stream>>>(base.keys, count, list, cap, out_idx,
+                                                                      out_dist);
+    check_launch("escalation scan kernels");
 }
 
 uint32_t scan_pairs_max_dim() { return 4096; }
