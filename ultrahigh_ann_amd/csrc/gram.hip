@@ -585,11 +585,12 @@ __device__ __forceinline__ double wave_distance_f64(const float* __restrict__ qr
 }
 
 // One block per query. See the file comment for the algorithm.
-// Exact re-rank of `count` candidate rows s_cand of query q (ROWS per pass
-// over the query row): binary32
-// distances in the reference's order (thread t is chain t) into s_dist, the
-// lowest (distance, row), then the binary64 minimum among candidates inside
-// the binary32 error band. Uses the whole block.
+// Exact re-rank of `count` candidate rows s_cand of query q: binary32
+// distances in the reference's order into s_dist (16-byte aligned rows: each
+// wave takes ROWS / 4 candidates per pass over the query row, four chains per
+// lane; otherwise thread t is chain t, ROWS candidates per pass), the lowest
+// (distance, row), then the binary64 minimum among the candidates inside the
+// binary32 error band (one candidate per wave). Uses the whole block.
 template <int ROWS>
 __device__ void rerank_pick(const SelectArgs& a, uint32_t q, const uint32_t* s_cand, uint32_t count, float* s_dist,
                             float (*s_group)[ROWS], double* s_d64, uint32_t* s_best, float* s_best_d,
