@@ -128,7 +128,10 @@ int uhann_scan_query_device_batch(const uhann_scan* scan, uhann_workspace* ws,
  * passed as void*; NULL = default stream). Queries have row stride
  * ld_queries >= dim; device_out_idx / device_out_dist (either may be NULL)
  * receive the results when the stream reaches them. No host
- * synchronisation. No counterpart in the reference (its calls are
+ * synchronisation in any strategy (the mixed strategy's escalations are
+ * handled on the device), so a call can be captured into a HIP graph once
+ * the workspace has run one batch of that strategy (its buffers are
+ * allocated on first use). No counterpart in the reference (its calls are
  * synchronous on the legacy stream): for callers that batch or capture. */
 int uhann_scan_query_device_async(const uhann_scan* scan, uhann_workspace* ws,
                                   const float* device_queries, uint64_t ld_queries,
