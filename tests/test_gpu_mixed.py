@@ -246,3 +246,34 @@ def test_gram_split_tail(uh, r, d, b):
         # estimates keep the whole-tile bits
         same = split == whole[:, :r].astype(np.float64)
         assert same.mean() > 0.8
+
+
+ODD_SHAPES = [(1, 70, 5), (2, 1, 3), (255, 63, 257), (256, 64, 256), (257, 65, 255), (511, 129, 2),
+              (33, 1000, 1), (600, 127, 513)]
+
+
+@pytest.mark.parametrize("r,d,b", ODD_SHAPES)
+def test_mixed_and_fp8_odd_shapes(uh, oracle_mod, r, d, b):
+    """Shapes around the 256-row Gram tiles, the 64-wide (BF16) and 128-wide
+    (FP8) K steps, one reference, one coordinate and one query: the BF16
+    strategy meets the same bar as the larger cases, and the FP8 strategy
+    returns a row whose distance is its binary32 reference distance."""
+    g = rng(r * 7 + d * 3 + b)
+    refs = (g.random((r, d), dtype=np.float32) * 4).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 4).astype(F32)
+    if r > 2:
+        refs[-1] = refs[0]
+        q[0] = refs[0]
+    _check_against_direct(uh, oracle_mod, refs, q)
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed(uh.STRATEGY_GRAM_FP8)
+    ws = index.make_query_workspace(b)
+    idx, dist = index.query_batch(q, b, ws, uh.STRATEGY_GRAM_FP8)
+    k2 = oracle_mod.k2_distances(refs, q)
+    rows = np.arange(b)
+    np.testing.assert_array_equal(dist.view(np.uint32), k2[rows, idx.astype(np.int64)].view(np.uint32))
+    if r <= 16:  # the shortlist holds every row: the binary32 argmin (or a binary64 near tie)
+        exp_idx, _ = oracle_mod.fp32_argmin(k2)
+        table = oracle_mod.fp64_table(False, refs, q)
+        same = idx == exp_idx
+        assert np.all(same | (table[rows, idx.astype(np.int64)] <= table[rows, exp_idx.astype(np.int64)]))
