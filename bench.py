@@ -86,6 +86,28 @@ def cpu_baseline(refs_np, queries_np, sample):
     }, idx
 
 
+def cpu_baseline_sequential(refs_np, queries_np, sample, expect_idx):
+    """The reference's published methodology: its `sequential` strategy on
+    one thread (oracle/_ref), on a few queries of the batch; None when the
+    reference build is absent."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    q = np.ascontiguousarray(queries_np[:sample])
+    r, d = refs_np.shape
+    idx = np.zeros(sample, np.uint64)
+    t0 = time.perf_counter()
+    rc = oracle.ref().ref_exact_l2(oracle._p(refs_np, oracle._f32p), r, d, oracle._p(q, oracle._f32p), sample, 0,
+                                   oracle._p(idx, oracle._u64p))
+    elapsed = time.perf_counter() - t0
+    if rc != 0:
+        return {"error": "reference sequential CPU scan failed"}
+    return {"value": sample / elapsed, "unit": "queries/s", "cores": 1, "kind": "reference",
+            "sample": f"{sample} queries, exact L2 binary64, sequential strategy, {elapsed:.2f} s wall",
+            "same_argmin_as_parallel": bool(np.array_equal(idx, expect_idx[:sample]))}
+
+
 def timed(fn, steps, warmup, torch):
     """Run fn() warmup + steps times; return (wall seconds, per-step event ms)."""
     for _ in range(warmup):
@@ -555,12 +577,14 @@ def main():
     }
 
     cpu = None
+    cpu_seq = None
     agreement = None
     if not args.no_cpu:
         threads = cpu_threads()
         sample = args.cpu_sample or max(8, 2 * threads)
         log(f"CPU baseline: {sample} queries on {threads} threads")
         cpu, fp64_idx = cpu_baseline(refs_np, queries.cpu().numpy(), sample)
+        cpu_seq = cpu_baseline_sequential(refs_np, queries.cpu().numpy(), 2, fp64_idx)
         agreement = {
             "sample": sample,
             "fp32_direct": float(np.mean(fp32_idx[:sample] == fp64_idx)),
@@ -610,6 +634,7 @@ def main():
         "agreement_fp64": agreement,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_sequential": cpu_seq,
         "workloads": workloads,
     }
     print(json.dumps(line), flush=True)
