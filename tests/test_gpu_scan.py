@@ -15,7 +15,7 @@ TILES = [(1, 8, 8, 8), (1, 12, 12, 8), (1, 16, 8, 4), (1, 8, 16, 2), (1, 4, 16, 
          (1, 1, 16, 1), (4, 1, 8, 1), (4, 2, 8, 1), (4, 4, 8, 1), (4, 8, 4, 2),
          (1, 1, 4, 1), (1, 2, 8, 1), (1, 4, 8, 1), (1, 16, 4, 4), (4, 1, 4, 1), (4, 2, 4, 1), (4, 4, 4, 1),
          (4, 1, 2, 1), (4, 1, 1, 1), (4, 2, 2, 1), (4, 4, 2, 1),
-         (0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0)]
+         (0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0), (0, 0, 0, 0)]
 
 
 def test_exact_l2_reference_fixture(uh):
@@ -134,7 +134,7 @@ def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
     dq = torch.from_numpy(q).cuda()
     oi = torch.zeros(b, dtype=torch.int64, device="cuda")
     od = torch.zeros(b, dtype=torch.float32, device="cuda")
-    if tile[0] == 4 and d % 4 != 0:
+    if (tile[0] == 4 and d % 4 != 0) or (tile[:2] == (0, 0) and d > 128):
         with pytest.raises(uh.UhannInvalidArgument):
             index.query_device_tile(dq, b, ws, oi, od, tile)
         return
@@ -142,6 +142,36 @@ def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
     np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+
+
+@pytest.mark.parametrize("d", [1, 5, 16, 17, 31, 32, 33, 63, 64, 65, 100, 127, 128])
+@pytest.mark.parametrize("r,b", [(1, 3), (130, 300), (300, 257)])
+def test_short_rows_bit_exact(uh, oracle_mod, d, r, b):
+    """The thread-per-query scan for rows of at most 128 coordinates (one term
+    per chain, empty chains and groups left out): L2, L1 and weighted L1
+    distances bit-identical to the oracle's replay of the reference's
+    kernels, through the forced tile and through the automatic choice
+    (batches of 256 or more queries)."""
+    import torch
+
+    refs, q = _data(d * 13 + r + b, r, d, b)
+    w = (rng(d + 7).random(d, dtype=np.float32) * 3 + 0.1).astype(F32)
+    cases = ((uh.HipDenseL2ScanIndex(refs), oracle_mod.k2_distances(refs, q)),
+             (uh.HipDenseL1ScanIndex(refs), oracle_mod.k5_distances(refs, q)),
+             (uh.HipDenseL1ScanIndex(refs, w), oracle_mod.k5_distances(refs, q, w, 0)))
+    dq = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    for index, expect in cases:
+        exp_idx, exp_dist = oracle_mod.fp32_argmin(expect)
+        ws = index.make_query_workspace(b)
+        index.query_device_tile(dq, b, ws, oi, od, (0, 0, 0, 0), stream=torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+        np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+        idx, dist = index.query_batch(q, b, ws)
+        np.testing.assert_array_equal(idx, exp_idx)
+        np.testing.assert_array_equal(dist.view(np.uint32), exp_dist.view(np.uint32))
 
 
 STREAM_TILES = [t for t in TILES if t[0] == 4]

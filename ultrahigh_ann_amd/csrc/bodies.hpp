@@ -268,6 +268,92 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
 }
 
 
+// Thread-per-query form for rows of at most 128 coordinates (hierarchical p,
+// small flat |I|): thread t owns query q0 + t with its row in registers, the
+// block's kShortRefs references (and the L1 weights) sit in LDS, zero-padded
+// to DMAX columns, and the thread walks them keeping its own (distance, row)
+// minimum, so a pair costs no cross-lane reduction. With at most 256 columns
+// every chain holds at most one term: chain c is the reference's first
+// accumulation step from +0 (fmaf(d, d, 0) for L2, 0 + |d| or the weighted
+// step for L1). The 32-chain trees (offsets 16, 8, 4, 2, 1) of the groups
+// that hold columns follow, then ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7)); chains
+// and groups without columns contribute +0, and adding +0 to a non-negative
+// sum changes no bit, so they are left out. Same distances as the
+// reference's K2/K5.
+constexpr int kShortRefs = 128;
+constexpr int kShortThreads = 256;
+
+template <int METRIC, bool WEIGHTED, bool WFMA, int DMAX>
+__device__ __forceinline__ void scan_short(const float* __restrict__ refs, uint64_t ld_ref, uint32_t rows,
+                                           const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
+                                           const float* __restrict__ weights, uint32_t dim,
+                                           unsigned long long* __restrict__ keys, uint32_t q0, uint32_t r0)
+{
+    static_assert(DMAX == 16 || DMAX == 32 || DMAX == 64 || DMAX == 128, "DMAX must be 16, 32, 64 or 128");
+    constexpr int NG = DMAX < 32 ? 1 : DMAX / 32;  // groups holding columns
+    constexpr int GW = DMAX < 32 ? DMAX : 32;      // chains per group that can hold a column
+    __shared__ __attribute__((aligned(16))) float ls[kShortRefs * DMAX];
+    __shared__ __attribute__((aligned(16))) float lw[DMAX];
+    const int t = threadIdx.x;
+    for (uint32_t e = t; e < static_cast<uint32_t>(kShortRefs * DMAX); e += kShortThreads) {
+        const uint32_t j = e / DMAX;
+        const uint32_t c = e % DMAX;
+        const uint32_t r = r0 + j;
+        ls[e] = (r < rows && c < dim) ? refs[static_cast<uint64_t>(r) * ld_ref + c] : 0.0F;
+    }
+    if constexpr (WEIGHTED) {
+        for (uint32_t c = t; c < static_cast<uint32_t>(DMAX); c += kShortThreads) lw[c] = c < dim ? weights[c] : 0.0F;
+    }
+    const uint32_t q = q0 + static_cast<uint32_t>(t);
+    const float* qrow = queries + static_cast<uint64_t>(min(q, nq - 1U)) * ld_q;
+    float qv[DMAX];
+#pragma unroll
+    for (int c = 0; c < DMAX; ++c) qv[c] = static_cast<uint32_t>(c) < dim ? qrow[c] : 0.0F;
+    __syncthreads();
+
+    unsigned long long best = ~0ULL;
+    const uint32_t nref = min(static_cast<uint32_t>(kShortRefs), rows - r0);
+    for (uint32_t j = 0; j < nref; ++j) {
+        const float* rj = ls + j * DMAX;
+        float W[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            float x[GW];
+#pragma unroll
+            for (int i = 0; i < GW; i += 4) {
+                const float4 rv = *reinterpret_cast<const float4*>(rj + 32 * g + i);
+                float4 wv = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+                if constexpr (WEIGHTED) wv = *reinterpret_cast<const float4*>(lw + 32 * g + i);
+                x[i] = accumulate<METRIC, WEIGHTED, WFMA>(0.0F, rv.x, qv[32 * g + i], wv.x);
+                x[i + 1] = accumulate<METRIC, WEIGHTED, WFMA>(0.0F, rv.y, qv[32 * g + i + 1], wv.y);
+                x[i + 2] = accumulate<METRIC, WEIGHTED, WFMA>(0.0F, rv.z, qv[32 * g + i + 2], wv.z);
+                x[i + 3] = accumulate<METRIC, WEIGHTED, WFMA>(0.0F, rv.w, qv[32 * g + i + 3], wv.w);
+            }
+            // Offsets 16 .. 1; with GW = 16 the offset-16 level only adds the
+            // empty chains 16 .. 31.
+#pragma unroll
+            for (int off = 16; off >= 1; off /= 2) {
+                if (off >= GW) continue;
+#pragma unroll
+                for (int i = 0; i < off; ++i) x[i] = x[i] + x[i + off];
+            }
+            W[g] = x[0];
+        }
+        float dist;
+        if constexpr (NG == 1) {
+            dist = W[0];
+        } else if constexpr (NG == 2) {
+            dist = W[0] + W[1];
+        } else {
+            dist = (W[0] + W[2]) + (W[1] + W[3]);
+        }
+        const unsigned long long key = (static_cast<unsigned long long>(__float_as_uint(dist)) << 32) | (r0 + j);
+        best = key < best ? key : best;
+    }
+    if (q < nq) atomicMin(keys + q, best);
+}
+
+
 // Pair-per-thread form for short rows (flat |I|, hierarchical p): the
 // 256-chain tree costs 255 adds per pair whatever the row length, and in the
 // tiled kernel above it is a cross-lane tree, which dominates when a chain

@@ -203,6 +203,7 @@ void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_
         if (scan.refs_t.bytes() != 0 && scan.dim != 0) {
             for (int tq : {1, 2, 4}) cands.push_back(kern::ScanTile{0, tq, 0, 0});
         }
+        if (scan.dim != 0 && scan.dim <= kern::scan_short_max_dim()) cands.push_back(kern::ScanTile{0, 0, 0, 0});
         for (uint64_t nq = 1;; nq *= 2) {
             const uint64_t n = std::min(nq, max_batch);
             float best = -1.0F;
@@ -244,6 +245,18 @@ void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_
     static_cast<void>(hipEventDestroy(e0));
     static_cast<void>(hipEventDestroy(e1));
 }
+
+namespace {
+
+uint32_t device_cus(int device)
+{
+    int cus = 0;
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
+              "cannot read the compute unit count");
+    return cus > 0 ? static_cast<uint32_t>(cus) : 0U;
+}
+
+}  // namespace
 
 kern::ScanTile choose_tile(uint64_t nq, bool aligned4)
 {
@@ -288,19 +301,32 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     // Short rows: the pair-per-thread kernel (tile.rpt == 0 selects it, tq
     // queries per thread) unless a tiled shape is forced or tuned.
     const bool pairs_ok = scan.refs_t.bytes() != 0 && a.dim != 0;
+    // Rows of at most 128 coordinates: one thread per query (tile.rpt == 0,
+    // tq == 0) when the launch has at least one 256-query x 128-reference
+    // block per compute unit; fewer pairs go to the pair-per-thread form.
+    const bool short_ok = a.dim != 0 && a.dim <= kern::scan_short_max_dim();
+    const uint64_t short_blocks = ((count + 255) / 256) * ((scan.rows + 127) / 128);
     const Scan::Tuned& tuned = scan.tuned[batch_bucket(count)];
     if (forced_tile == nullptr && tuned.set && (tuned.tile.rpt <= 1 || aligned4)) {
         tile = tuned.tile;
+    } else if (forced_tile == nullptr && short_ok && short_blocks >= device_cus(scan.device)) {
+        tile = kern::ScanTile{0, 0, 0, 0};
     } else if (forced_tile == nullptr && pairs_ok && a.dim <= pairs_max_dim() && count >= 4) {
         tile = kern::ScanTile{0, 4, 0, 0};
     }
     if (tile.rpt == 0) {
-        if (!pairs_ok) throw std::invalid_argument("pair scan needs rows of at most 4096 coordinates");
+        const kern::Metric metric = scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1;
+        if (tile.tq == 0 && !short_ok) throw std::invalid_argument("short-row scan needs 1 to 128 coordinates");
+        if (tile.tq != 0 && !pairs_ok) throw std::invalid_argument("pair scan needs rows of at most 4096 coordinates");
         kern::fill_keys(keys, a.nq, stream);
         if (timed != nullptr) timed->mark(0, stream);
-        kern::scan_pairs(scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1,
-                         /*weight_fma=*/true, scan.refs_t.data(), scan.rows, a.rows, queries, ld_q, a.nq,
-                         a.weights, a.dim, keys, tile.tq, stream);
+        if (tile.tq == 0) {
+            kern::scan_short(metric, /*weight_fma=*/true, scan.refs.data(), scan.ld, a.rows, queries, ld_q, a.nq,
+                             a.weights, a.dim, keys, stream);
+        } else {
+            kern::scan_pairs(metric, /*weight_fma=*/true, scan.refs_t.data(), scan.rows, a.rows, queries, ld_q, a.nq,
+                             a.weights, a.dim, keys, tile.tq, stream);
+        }
         if (timed != nullptr) timed->mark(1, stream);
         kern::finalize_keys(keys, a.nq, out_idx, out_dist, stream);
         if (timed != nullptr && timed->timing) {
@@ -399,18 +425,6 @@ void Scan::enable_mixed(int strategy)
     payload_bytes += g->refs16.bytes() + g->stats.bytes();
     gram = std::move(g);
 }
-
-namespace {
-
-uint32_t device_cus(int device)
-{
-    int cus = 0;
-    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
-              "cannot read the compute unit count");
-    return cus > 0 ? static_cast<uint32_t>(cus) : 0U;
-}
-
-}  // namespace
 
 void Workspace::reserve_partials(uint64_t floats)
 {
@@ -773,7 +787,7 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
         checked_ws(scan, ws, count);
         const kern::ScanTile tile{rpt, tq, tr, group_q};
-        const bool pair_tile = rpt == 0 && (tq == 1 || tq == 2 || tq == 4);
+        const bool pair_tile = rpt == 0 && (tq == 0 || tq == 1 || tq == 2 || tq == 4);
         if (!pair_tile && !kern::scan_tile_supported(tile)) throw std::invalid_argument("unsupported scan tile");
         if (count == 0) return;
         if (scan->dim != 0 && device_queries == nullptr) {

@@ -74,6 +74,14 @@ void finalize_keys(const unsigned long long* keys, uint32_t n, uint64_t* out_idx
 void escalate_scan(const ScanArgs& base, const unsigned int* count, const unsigned int* list, uint32_t cap,
                    float* staged, uint64_t ld_staged, uint64_t* out_idx, float* out_dist, hipStream_t stream);
 
+// Thread-per-query direct scan for 1 <= dim <= scan_short_max_dim(): refs
+// row-major (row stride ld_ref), one thread per query walking 128 references
+// staged in LDS. Same keys contract as scan_direct, same bits.
+uint32_t scan_short_max_dim();
+void scan_short(Metric metric, bool weight_fma, const float* refs, uint64_t ld_ref, uint32_t rows,
+                const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,
+                unsigned long long* keys, hipStream_t stream);
+
 // Pair-per-thread direct scan for dim <= scan_pairs_max_dim(): refs_t is the
 // transposed (dim x rows, row stride ld_t) reference matrix; tq (1, 2, 4)
 // queries per block. Same keys contract as scan_direct, same bits.

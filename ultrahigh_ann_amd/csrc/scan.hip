@@ -132,6 +132,18 @@ escalate_finalize_kernel(const unsigned long long* __restrict__ keys, const unsi
     }
 }
 
+// Thread-per-query form for short rows (body::scan_short): grid.x covers
+// the references kShortRefs at a time, grid.y the queries 256 at a time.
+template <int METRIC, bool WEIGHTED, bool WFMA, int DMAX>
+__global__ void __launch_bounds__(body::kShortThreads)
+scan_short_kernel(const float* __restrict__ refs, uint64_t ld_ref, uint32_t rows, const float* __restrict__ queries,
+                  uint64_t ld_q, uint32_t nq, const float* __restrict__ weights, uint32_t dim,
+                  unsigned long long* __restrict__ keys)
+{
+    body::scan_short<METRIC, WEIGHTED, WFMA, DMAX>(refs, ld_ref, rows, queries, ld_q, nq, weights, dim, keys,
+                                                   blockIdx.y * body::kShortThreads, blockIdx.x * body::kShortRefs);
+}
+
 __global__ void fill_keys_kernel(unsigned long long* keys, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -263,6 +275,40 @@ void escalate_scan(const ScanArgs& base, const unsigned int* count, const unsign
     escalate_finalize_kernel<<<(cap + 255U) / 256U, 256, 0, stream>>>(base.keys, count, list, cap, out_idx,
                                                                       out_dist);
     check_launch("escalation scan kernels");
+}
+
+uint32_t scan_short_max_dim() { return 128; }
+
+void scan_short(Metric metric, bool weight_fma, const float* refs, uint64_t ld_ref, uint32_t rows,
+                const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,
+                unsigned long long* keys, hipStream_t stream)
+{
+    if (nq == 0 || rows == 0) return;
+    if (dim == 0 || dim > scan_short_max_dim()) throw std::invalid_argument("short-row scan supports 1 <= dim <= 128");
+    const dim3 grid((rows + body::kShortRefs - 1) / body::kShortRefs, (nq + body::kShortThreads - 1) / body::kShortThreads);
+    if (grid.y > 65535U) throw std::length_error("short-row scan batch exceeds the grid limit");
+    const bool weighted = weights != nullptr;
+    const int m = metric == Metric::l2 ? 0 : 1;
+#define UHANN_SHORT(M, W, F, D)                                                                                 \
+    hipLaunchKernelGGL((scan_short_kernel<M, W, F, D>), grid, dim3(body::kShortThreads), 0, stream, refs, ld_ref, \
+                       rows, queries, ld_q, nq, weights, dim, keys)
+#define UHANN_SHORT_D(D)                                 \
+    if (m == 0) UHANN_SHORT(0, false, false, D);         \
+    else if (!weighted) UHANN_SHORT(1, false, false, D); \
+    else if (weight_fma) UHANN_SHORT(1, true, true, D);  \
+    else UHANN_SHORT(1, true, false, D);
+    if (dim <= 16) {
+        UHANN_SHORT_D(16)
+    } else if (dim <= 32) {
+        UHANN_SHORT_D(32)
+    } else if (dim <= 64) {
+        UHANN_SHORT_D(64)
+    } else {
+        UHANN_SHORT_D(128)
+    }
+#undef UHANN_SHORT_D
+#undef UHANN_SHORT
+    check_launch("short-row scan kernel");
 }
 
 uint32_t scan_pairs_max_dim() { return 4096; }
