@@ -142,9 +142,10 @@ int uhann_scan_query_device_async(const uhann_scan* scan, uhann_workspace* ws,
  * chains per thread (1, 2 or 4), tq x tr (query, reference) pairs per block,
  * group_q query tiles per reference sweep; rpt = 0 selects the
  * pair-per-thread scan for rows of at most 4096 coordinates with tq (1, 2, 4)
- * queries per thread. Returns INVALID_ARGUMENT for a tile that is not
- * compiled in. No counterpart in the reference: used by the
- * autotuner and by the tests that cover every tile. */
+ * queries per thread, and rpt = 0, tq = 0 the thread-per-query scan for rows
+ * of 1 to 128 coordinates. Returns INVALID_ARGUMENT for a tile that is not
+ * compiled in or does not fit the rows. No counterpart in the reference:
+ * used by the autotuner and by the tests that cover every tile. */
 int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
                                  const float* device_queries, uint64_t ld_queries, uint64_t count,
                                  uint64_t* device_out_idx, float* device_out_dist, int rpt, int tq,
