@@ -607,6 +607,32 @@ __device__ __forceinline__ void project_bf16_tile(const float* __restrict__ x, u
     }
 }
 
+// Batcher's odd-even merge sort (ascending) as a compile-time network of
+// compare-exchanges in registers: 63 / 191 / 543 / 1471 for N = 16 / 32 /
+// 64 / 128, against the bitonic network's 80 / 240 / 672 / 1792.
+template <int N>
+__device__ __forceinline__ void oddeven_merge_sort(float (&v)[N])
+{
+#pragma unroll
+    for (int p = 1; p < N; p *= 2) {
+#pragma unroll
+        for (int k = p; k >= 1; k /= 2) {
+#pragma unroll
+            for (int j = k % p; j + k < N; j += 2 * k) {
+#pragma unroll
+                for (int i = 0; i < k; ++i) {
+                    if (i < N - j - k && (i + j) / (2 * p) == (i + j + k) / (2 * p)) {
+                        const float a = v[i + j];
+                        const float b = v[i + j + k];
+                        v[i + j] = fminf(a, b);
+                        v[i + j + k] = fmaxf(a, b);
+                    }
+                }
+            }
+        }
+    }
+}
+
 template <int N>
 __device__ __forceinline__ void bitonic_sort(float (&v)[N])
 {
@@ -667,7 +693,7 @@ __device__ __forceinline__ void median_block(const float* __restrict__ refs_t, u
                 v[c] = __builtin_inff();
             }
         }
-        bitonic_sort<PMAX>(v);
+        oddeven_merge_sort<PMAX>(v);
         float med = 0.0F;
         if constexpr (EXACT) {
             // Fixed middle positions: the compiler keeps only the
