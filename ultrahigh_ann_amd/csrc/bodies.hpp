@@ -484,14 +484,34 @@ __device__ __forceinline__ void gather_tile(const float* __restrict__ src, uint6
     bool bad = false;
     const uint32_t q_begin = row_block * kGatherRows;
     const uint32_t q_end = min(q_begin + kGatherRows, nq);
-    for (uint32_t q = q_begin; q < q_end; ++q) {
-        const float* row = src + static_cast<uint64_t>(q) * ld_src;
+    // All of the thread's gathered values (kGatherRows rows x kPer columns)
+    // are loaded before any is stored, so their loads are in flight together.
+    constexpr int kPer = kGatherChunk / kGatherThreads;
+    float v[kGatherRows][kPer];
+#pragma unroll
+    for (int rq = 0; rq < kGatherRows; ++rq) {
+        const uint32_t q = q_begin + static_cast<uint32_t>(rq);
+        const float* row = src + static_cast<uint64_t>(min(q, nq - 1U)) * ld_src;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const uint32_t i = threadIdx.x + static_cast<uint32_t>(u) * kGatherThreads;
+            v[rq][u] = (q < q_end && i < n) ? row[s_cols[i]] : 0.0F;
+        }
+    }
+#pragma unroll
+    for (int rq = 0; rq < kGatherRows; ++rq) {
+        const uint32_t q = q_begin + static_cast<uint32_t>(rq);
+        if (q >= q_end) break;
         float* dst = out + static_cast<uint64_t>(q) * ld_out + c0;
-        for (uint32_t i = threadIdx.x; i < n; i += kGatherThreads) {
-            const float v = row[s_cols[i]];
-            const float scaled = scales != nullptr ? v * s_scales[i] : v;
-            bad |= !__builtin_isfinite(v) || !__builtin_isfinite(scaled);
-            dst[i] = scaled;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const uint32_t i = threadIdx.x + static_cast<uint32_t>(u) * kGatherThreads;
+            if (i < n) {
+                const float x = v[rq][u];
+                const float scaled = scales != nullptr ? x * s_scales[i] : x;
+                bad |= !__builtin_isfinite(x) || !__builtin_isfinite(scaled);
+                dst[i] = scaled;
+            }
         }
     }
     if (bad) atomicExch(invalid, 1);
