@@ -16,6 +16,9 @@
 namespace uhann::kern::body {
 
 constexpr int kChains = 256;
+// Word 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
+// data format, no swizzle.
+constexpr int kBufferWord3 = 0x00020000;
 constexpr int kGroups = 8;  // groups of 32 chains
 
 
@@ -354,7 +357,188 @@ __device__ __forceinline__ void scan_short(const float* __restrict__ refs, uint6
 }
 
 
+// A wave-uniform 64-bit value moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(x >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Terms base .. base+31 of the 32 chains of one group (chain base%256 + i
+// takes column base + i), for TQ queries, added to v. Columns at or past dim
+// leave their chain unchanged. At TQ <= 2, chunks that lie wholly below dim
+// skip the per-column checks. The reference column is read through a buffer
+// resource at the group's first column: scalar column offset plus the
+// thread's 32-bit byte offset roff.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__device__ __forceinline__ void pair_group_terms(float (&v)[TQ][32], const float* __restrict__ refs_t, uint64_t ld_t,
+                                                 uint32_t roff, const float* qs, uint32_t ldqs, const float* ws,
+                                                 uint32_t base, uint32_t dim)
+{
+    // Column base + i sits i * ld_t * 4 bytes past the resource base; with
+    // rows < 2^25 (ld_t == rows) every offset stays below 2^32.
+    const __amdgpu_buffer_rsrc_t cols = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(refs_t + static_cast<uint64_t>(base) * ld_t), 0, -1, kBufferWord3);
+    const uint32_t col_bytes = static_cast<uint32_t>(ld_t) * 4U;
+    float rv[32];
+    // At TQ = 4 the unchecked form needs more registers than two waves per
+    // SIMD allow, so it keeps the checked form.
+    if (TQ <= 2 && base + 32U <= dim) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            rv[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(cols, roff, i * col_bytes, 0));
+        }
+        // Eight columns at a time; the fence keeps each chunk's query reads
+        // next to their use.
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int i = 8 * j + k;
+                const float wv = WEIGHTED ? ws[base + i] : 0.0F;
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) {
+                    v[q][i] = accumulate<METRIC, WEIGHTED, WFMA>(v[q][i], rv[i], qs[q * ldqs + base + i], wv);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const uint32_t c = min(base + i, dim - 1U);
+            rv[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(cols, roff, (c - base) * col_bytes, 0));
+        }
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const uint32_t c = base + i;
+            const float wv = WEIGHTED && c < dim ? ws[c] : 0.0F;
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) {
+                if (c < dim) v[q][i] = accumulate<METRIC, WEIGHTED, WFMA>(v[q][i], rv[i], qs[q * ldqs + c], wv);
+            }
+        }
+    }
+}
+
 // Pair-per-thread form for short rows (flat |I|, hierarchical p): the
+// 256-chain tree costs 255 adds per pair whatever the row length, and in the
+// tiled kernel above it is a cross-lane tree, which dominates when a chain
+// holds one or two terms. Here each thread owns one reference row and TQ
+// queries and evaluates the whole reference order in registers: for each
+// group w of 32 chains, chain 32w+i sums columns 32w+i, 32w+i+256, ... in
+// order, then the 32-lane tree (offsets 16..1) gives W_w, and
+// ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7)) the distance -- the same additions as
+// the reference, so the same bits. A group with no columns (32w >= dim) has
+// W_w = +0, and adding +0 to a sum of non-negative terms changes no bit, so
+// it is skipped. References are read from a transposed (dim x rows) copy, so
+// a wave's loads are contiguous (rows < 2^25, so a group's 32 columns lie
+// within 2^32 bytes); queries (and L1 weights) sit in LDS with a row stride padded to a
+// multiple of four and are read as broadcast 16-byte vectors.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__device__ __forceinline__ void scan_pairs(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
+                                           const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
+                                           const float* __restrict__ weights, uint32_t dim,
+                                           unsigned long long* __restrict__ keys, uint32_t q0, uint32_t rblock,
+                                           float* smem)
+{
+    // The buffer resource and the column offsets must be scalar (a caller
+    // that read these from memory would otherwise get a per-lane loop around
+    // every load); they are uniform in every caller.
+    refs_t = reinterpret_cast<const float*>(uniform_u64(reinterpret_cast<uint64_t>(refs_t)));
+    ld_t = uniform_u64(ld_t);
+    rows = __builtin_amdgcn_readfirstlane(rows);
+    dim = __builtin_amdgcn_readfirstlane(dim);
+    const uint32_t ldqs = (dim + 3U) & ~3U;
+    float* qs = smem;               // TQ x ldqs
+    float* ws = smem + TQ * ldqs;   // ldqs (weighted L1)
+    const int t = threadIdx.x;
+    for (uint32_t e = t; e < TQ * ldqs; e += 256) {
+        const uint32_t qi = e / ldqs;
+        const uint32_t c = e % ldqs;
+        qs[e] = c < dim ? queries[static_cast<uint64_t>(min(q0 + qi, nq - 1U)) * ld_q + c] : 0.0F;
+    }
+    if constexpr (WEIGHTED) {
+        for (uint32_t c = t; c < ldqs; c += 256) ws[c] = c < dim ? weights[c] : 0.0F;
+    }
+    __syncthreads();
+    const uint32_t r = rblock * 256 + t;
+    const uint32_t roff = min(r, rows - 1U) * 4U;
+    // Groups are visited in bit-reversed order 0,4 | 2,6 | 1,5 | 3,7, so pass
+    // p of 4 produces (W0+W4), (W2+W6), (W1+W5), (W3+W7) in turn and the
+    // distance is (P0 + P1) + (P2 + P3): the reference's final tree with only
+    // four partial sums live (the loops stay rolled to keep registers low).
+    float P[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[q][k] = 0.0F;
+#pragma unroll 1
+    for (int pass = 0; pass < 4; ++pass) {
+        float pair_sum[TQ];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const int g = 2 * pass + h;
+            const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
+            const uint32_t gbase = static_cast<uint32_t>(32 * w);
+            if (gbase >= dim) {
+                if (h == 0) {
+#pragma unroll
+                    for (int q = 0; q < TQ; ++q) pair_sum[q] = 0.0F;
+                }
+                continue;
+            }
+            float v[TQ][32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) v[q][i] = 0.0F;
+            // Term k of chain 32w+i is column 32w + i + 256k: all 32 chains of
+            // the group load term k together (32 independent loads in flight),
+            // and each chain still adds its terms in increasing k.
+            for (uint32_t base = gbase; base < dim; base += kChains) {
+                pair_group_terms<METRIC, WEIGHTED, WFMA, TQ>(v, refs_t, ld_t, roff, qs, ldqs, ws, base, dim);
+            }
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) {
+#pragma unroll
+                for (int off = 16; off >= 1; off /= 2) {
+#pragma unroll
+                    for (int i = 0; i < off; ++i) v[q][i] = v[q][i] + v[q][i + off];
+                }
+                pair_sum[q] = h == 0 ? v[q][0] : pair_sum[q] + v[q][0];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) P[q][k] = pass == k ? pair_sum[q] : P[q][k];
+    }
+    __shared__ unsigned long long best[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+        const float dist = (P[q][0] + P[q][1]) + (P[q][2] + P[q][3]);
+        unsigned long long key = r < rows
+                                     ? (static_cast<unsigned long long>(__float_as_uint(dist)) << 32) | r
+                                     : ~0ULL;
+        for (int off = 32; off >= 1; off /= 2) {
+            const unsigned long long o = __shfl_down(key, off);
+            key = o < key ? o : key;
+        }
+        if (t % 64 == 0) best[q][t / 64] = key;
+    }
+    __syncthreads();
+    if (t < TQ && q0 + t < nq) {
+        unsigned long long b = best[t][0];
+        for (int w = 1; w < 4; ++w) b = best[t][w] < b ? best[t][w] : b;
+        atomicMin(keys + q0 + t, b);
+    }
+}
+
+// Pair-per-thread form used by the serving kernel (rows of many small
+// indexes, read through global loads; measured faster there than the
+// buffer-load form below). Same chains and tree as scan_pairs: the
 // 256-chain tree costs 255 adds per pair whatever the row length, and in the
 // tiled kernel above it is a cross-lane tree, which dominates when a chain
 // holds one or two terms. Here each thread owns one reference row and TQ
@@ -366,7 +550,7 @@ __device__ __forceinline__ void scan_short(const float* __restrict__ refs, uint6
 // (dim x rows) copy, so a wave's loads are contiguous; queries (and L1
 // weights) sit in LDS and are broadcast.
 template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
-__device__ __forceinline__ void scan_pairs(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
+__device__ __forceinline__ void scan_pairs_serve(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
                                            const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
                                            const float* __restrict__ weights, uint32_t dim,
                                            unsigned long long* __restrict__ keys, uint32_t q0, uint32_t rblock,

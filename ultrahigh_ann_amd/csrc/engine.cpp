@@ -300,7 +300,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     kern::ScanTile tile = forced_tile != nullptr ? *forced_tile : choose_tile(count, aligned4);
     // Short rows: the pair-per-thread kernel (tile.rpt == 0 selects it, tq
     // queries per thread) unless a tiled shape is forced or tuned.
-    const bool pairs_ok = scan.refs_t.bytes() != 0 && a.dim != 0;
+    const bool pairs_ok = scan.refs_t.bytes() != 0 && a.dim != 0 && scan.rows < kern::kPairsMaxRows;
     // Rows of at most 128 coordinates: one thread per query (tile.rpt == 0,
     // tq == 0) when the launch has at least one 256-query x 128-reference
     // block per compute unit; fewer pairs go to the pair-per-thread form.
@@ -312,7 +312,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     } else if (forced_tile == nullptr && short_ok && short_blocks >= device_cus(scan.device)) {
         tile = kern::ScanTile{0, 0, 0, 0};
     } else if (forced_tile == nullptr && pairs_ok && a.dim <= pairs_max_dim() && count >= 4) {
-        tile = kern::ScanTile{0, 4, 0, 0};
+        tile = kern::ScanTile{0, 2, 0, 0};
     }
     if (tile.rpt == 0) {
         const kern::Metric metric = scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1;

@@ -84,7 +84,9 @@ void scan_short(Metric metric, bool weight_fma, const float* refs, uint64_t ld_r
 
 // Pair-per-thread direct scan for dim <= scan_pairs_max_dim(): refs_t is the
 // transposed (dim x rows, row stride ld_t) reference matrix; tq (1, 2, 4)
-// queries per block. Same keys contract as scan_direct, same bits.
+// queries per block, rows < kPairsMaxRows. Same keys contract as
+// scan_direct, same bits.
+constexpr uint32_t kPairsMaxRows = 1U << 25;
 uint32_t scan_pairs_max_dim();
 void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld_t, uint32_t rows,
                 const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,

@@ -320,7 +320,9 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     if (nq == 0 || rows == 0) return;
     if (dim > scan_pairs_max_dim()) throw std::invalid_argument("pair scan supports dim <= 4096");
     const bool weighted = weights != nullptr;
-    const size_t lds = (static_cast<size_t>(tq) * dim + (weighted ? dim : 0)) * sizeof(float);
+    if (rows >= kPairsMaxRows) throw std::length_error("pair scan supports fewer than 2^25 reference rows");
+    const size_t ldqs = (static_cast<size_t>(dim) + 3) & ~static_cast<size_t>(3);
+    const size_t lds = (static_cast<size_t>(tq) * ldqs + (weighted ? ldqs : 0)) * sizeof(float);
     const dim3 grid((rows + 255) / 256, (nq + tq - 1) / tq);
     if (grid.y > 65535U) throw std::length_error("pair scan batch exceeds the grid limit");
 #define UHANN_PAIRS(M, W, F, Q)                                                                   \

@@ -139,8 +139,8 @@ serve_pairs_kernel(const void* blob, int stage, unsigned long long* keys)
         const Unit w = locate(blob, stage, u);
         const ServeItem& it = *w.item;
         __syncthreads();
-        body::scan_pairs<METRIC, WEIGHTED, true, TQ>(it.refs_t, it.rows, it.rows, it.scan_q, it.ld_sq, it.n,
-                                                      it.weights, it.dim, keys + it.qoff, w.b * TQ, w.a, smem);
+        body::scan_pairs_serve<METRIC, WEIGHTED, true, TQ>(it.refs_t, it.rows, it.rows, it.scan_q, it.ld_sq, it.n,
+                                                            it.weights, it.dim, keys + it.qoff, w.b * TQ, w.a, smem);
     }
 }
 
