@@ -586,6 +586,13 @@ __device__ __forceinline__ void scan_pairs_serve(const float* __restrict__ refs_
         for (int h = 0; h < 2; ++h) {
             const int g = 2 * pass + h;
             const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
+            if (static_cast<uint32_t>(32 * w) >= dim) {  // no columns: W_w = +0 changes no bit
+                if (h == 0) {
+#pragma unroll
+                    for (int q = 0; q < TQ; ++q) pair_sum[q] = 0.0F;
+                }
+                continue;
+            }
             float v[TQ][32];
 #pragma unroll
             for (int i = 0; i < 32; ++i)
