@@ -317,7 +317,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     if (tile.rpt == 0) {
         const kern::Metric metric = scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1;
         if (tile.tq == 0 && !short_ok) throw std::invalid_argument("short-row scan needs 1 to 128 coordinates");
-        if (tile.tq != 0 && !pairs_ok) throw std::invalid_argument("pair scan needs rows of at most 4096 coordinates");
+        if (tile.tq != 0 && !pairs_ok) throw std::invalid_argument("pair scan needs rows of at most 4096 coordinates and fewer than 2^25 references");
         kern::fill_keys(keys, a.nq, stream);
         if (timed != nullptr) timed->mark(0, stream);
         if (tile.tq == 0) {
