@@ -144,6 +144,38 @@ def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
     np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
 
 
+@pytest.mark.parametrize("tq", [1, 2, 4])
+@pytest.mark.parametrize("metric", ["l2", "l1", "l1w"])
+@pytest.mark.parametrize("d", [222, 300, 32])
+def test_pair_tiles_many_reference_blocks(uh, oracle_mod, tq, metric, d):
+    # Pair-per-thread scan over several 256-row reference blocks (the last one
+    # partial), whole 32-column chunks and tails, empty groups (d = 32, 222),
+    # and the L1 weights, for every queries-per-thread width.
+    import torch
+
+    r, b = 600, 9
+    refs, q = _data(11 * d + tq, r, d, b)
+    if metric == "l2":
+        index = uh.HipDenseL2ScanIndex(refs)
+        expect = oracle_mod.k2_distances(refs, q)
+    elif metric == "l1":
+        index = uh.HipDenseL1ScanIndex(refs)
+        expect = oracle_mod.k5_distances(refs, q)
+    else:
+        w = (rng(d).random(d, dtype=np.float32) * 3 + 0.1).astype(F32)
+        index = uh.HipDenseL1ScanIndex(refs, w)
+        expect = oracle_mod.k5_distances(refs, q, w, 0)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(expect)
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    index.query_device_tile(dq, b, ws, oi, od, (0, tq, 0, 0), stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+
+
 @pytest.mark.parametrize("d", [1, 5, 16, 17, 31, 32, 33, 63, 64, 65, 100, 127, 128])
 @pytest.mark.parametrize("r,b", [(1, 3), (130, 300), (300, 257)])
 def test_short_rows_bit_exact(uh, oracle_mod, d, r, b):
