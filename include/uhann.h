@@ -45,10 +45,13 @@ extern "C" {
  * cuda_dense_l2_scan_index.hpp:17-23). */
 #define UHANN_STRATEGY_DIRECT 0
 /* L2 only. ||q||^2 + ||v||^2 - 2 q.v with the inner product in BF16 on MFMA
- * (FP32 accumulation), a per-query candidate set bounded by the BF16 error,
- * an exact re-rank of the candidates in the direct path's binary32 order and
- * a binary64 re-rank of binary32 near ties. New (the reference's GEMM
- * strategy, cuda_dense_l2_scan_index.hpp:21-22, is unrefined FP32 SGEMM). */
+ * (FP32 accumulation), a per-query candidate set, an exact re-rank of the
+ * candidates in the direct path's binary32 order and a binary64 re-rank of
+ * binary32 near ties. The candidate bound is statistical (it models the
+ * bf16 rounding errors as independent): not a certificate; inputs with many
+ * repeated values can exceed it. GRAM_INT8 is the certified form. New (the
+ * reference's GEMM strategy, cuda_dense_l2_scan_index.hpp:21-22, is
+ * unrefined FP32 SGEMM). */
 #define UHANN_STRATEGY_GRAM_BF16 1
 /* Hierarchical L2 only. The |I| x p query projection runs as a GEMM in BF16
  * on MFMA (FP32 accumulation) instead of the reference's binary32 K7 order
@@ -63,6 +66,16 @@ extern "C" {
  * queries whose answer the FP8 error bound proves. New (the reference has no
  * FP8 path). */
 #define UHANN_STRATEGY_GRAM_FP8 3
+/* L2 only, dim <= 65536. Rows as 16-bit fixed point x^ = s (a + b / 256)
+ * (per-row binary32 scale s, int8 planes a and b); <q^, v^> from three int8
+ * MFMA products summed exactly in int32, combined in binary64. The candidate
+ * set comes from a deterministic bound (Cauchy-Schwarz on each row's actual
+ * quantisation error, rounded up), so it holds the direct path's answer for
+ * every input; candidates are re-ranked in the direct path's binary32 order
+ * and binary32 near ties in binary64; more than 64 candidates send the query
+ * to the direct scan on the device. New (the reference has no integer
+ * path). */
+#define UHANN_STRATEGY_GRAM_INT8 4
 
 /* Index kinds reported by uhann_index_info. */
 #define UHANN_KIND_EXACT 0
@@ -153,8 +166,9 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
                                  int tr, int group_q, void* stream);
 
 /* Builds the reference copy and row statistics the GRAM_BF16 strategy (rows x
- * dim bf16 + 32 bytes per row of device memory) or the GRAM_FP8 strategy
- * (rows x dim e4m3 bytes + 32 bytes per row) needs. Call before sharing the
+ * dim bf16 + 32 bytes per row of device memory), the GRAM_FP8 strategy
+ * (rows x dim e4m3 bytes + 32 bytes per row) or the GRAM_INT8 strategy (rows
+ * x 2 dim bytes + 32 bytes per row; dim > 65536 -> LENGTH) needs. Call before sharing the
  * handle between threads. L2 only. No counterpart in the
  * reference (its GEMM strategy needs no preparation beyond K1 norms,
  * cuda_dense_l2_scan_index.cu:449-462). */
@@ -236,7 +250,7 @@ void uhann_index_destroy(uhann_index* index);
 int uhann_index_info(const uhann_index* index, int* kind, int* metric, uint64_t* rows, uint64_t* dim,
                      uint64_t* unique_coordinates, uint64_t* multiplicity, uint64_t* payload_bytes);
 
-/* Prepares GRAM_BF16 or GRAM_FP8 for an exact or flat L2 index (see
+/* Prepares GRAM_BF16, GRAM_FP8 or GRAM_INT8 for an exact or flat L2 index (see
  * uhann_scan_enable_mixed). */
 int uhann_index_enable_mixed(uhann_index* index, int strategy);
 
@@ -378,6 +392,27 @@ int uhann_scan_distance_table(const uhann_scan* scan, const float* device_querie
 int uhann_scan_fp8_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
                              uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out,
                              void* stream);
+
+/* The GRAM_INT8 strategy's estimates ||q||^2 + ||v||^2 - 2 s_q s_v (H + X /
+ * 256) (exact int32 sums, binary64 combination, rounded to binary32) for
+ * count device queries against every reference, as uhann_scan_gram_estimates
+ * (tile -1: the default shape with its K split; 0 .. 3 force a block shape,
+ * every tile whole; all give the same bits, since the sums are exact). Needs
+ * uhann_scan_enable_mixed(GRAM_INT8). No counterpart in the reference: for
+ * diagnostics and tests. */
+int uhann_scan_int8_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                              uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out, int tile,
+                              void* stream);
+
+/* The GRAM_INT8 row conversion alone: count device rows (stride ld_rows,
+ * dim <= 65536 columns) into device_bytes (count rows of 128 ceil(dim / 64)
+ * bytes: per 64 columns, 64 bytes of the high plane a, then 64 of the low
+ * plane b) and device_stats (count records of 32 bytes: binary64 ||x||^2,
+ * then binary32 s, ||x - x^||, ||x^||, s ||b|| / 256, ||x||, padding). No
+ * counterpart in the reference: lets the tests check the codes and the
+ * rounded-up norms the certificate is built from. */
+int uhann_int8_rows(const float* device_rows, uint64_t ld_rows, uint64_t count, uint64_t dim, uint8_t* device_bytes,
+                    void* device_stats, void* stream);
 
 /* Autotuner of the direct scan: times every compiled kernel shape (the
  * tiled register tiles and the pair-per-thread scan) with HIP events for

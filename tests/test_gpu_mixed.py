@@ -12,43 +12,14 @@ one."""
 import numpy as np
 import pytest
 
-from conftest import bf16_round, rng
+from conftest import bf16_round, check_against_direct, rng
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
 
 
 def _check_against_direct(uh, oracle_mod, refs, q):
-    import torch
-
-    b = q.shape[0]
-    index = uh.HipExactL2Index(refs)
-    index.enable_mixed()
-    ws = index.make_query_workspace(b)
-    d_idx, d_dist = index.query_batch(q, b, ws, uh.STRATEGY_DIRECT)
-    m_idx, m_dist = index.query_batch(q, b, ws, uh.STRATEGY_GRAM_BF16)
-    stats = ws.mixed_stats()
-    # device-query path gives the same answer
-    dm_idx, dm_dist = index.query_device_batch(torch.from_numpy(q).cuda(), b, ws, uh.STRATEGY_GRAM_BF16)
-    np.testing.assert_array_equal(dm_idx, m_idx)
-    np.testing.assert_array_equal(dm_dist, m_dist)
-    k2 = oracle_mod.k2_distances(refs, q)
-    exp_idx, exp_dist = oracle_mod.fp32_argmin(k2)
-    np.testing.assert_array_equal(d_idx, exp_idx)
-    table = oracle_mod.fp64_table(False, refs, q)
-    fp64_idx = table.argmin(1)
-    rows = np.arange(b)
-    diff = m_idx != d_idx
-    # same row -> same binary32 distance bit for bit
-    np.testing.assert_array_equal(m_dist[~diff].view(np.uint32), d_dist[~diff].view(np.uint32))
-    # mixed distance is always the binary32 (reference-order) distance of its row
-    np.testing.assert_array_equal(m_dist.view(np.uint32), k2[rows, m_idx.astype(np.int64)].view(np.uint32))
-    # a different row only through the binary64 near-tie re-rank, never worse in binary64
-    assert np.all(table[rows[diff], m_idx[diff].astype(np.int64)] <= table[rows[diff], d_idx[diff].astype(np.int64)])
-    agree_mixed = np.mean(m_idx == fp64_idx)
-    agree_fp32 = np.mean(d_idx == fp64_idx)
-    assert agree_mixed >= agree_fp32
-    return stats, agree_mixed, agree_fp32
+    return check_against_direct(uh, oracle_mod, refs, q, uh.STRATEGY_GRAM_BF16)
 
 
 @pytest.mark.parametrize("r,d,b", [(300, 3000, 40), (129, 257, 130), (1000, 8192, 64), (7, 65, 3)])

@@ -69,6 +69,7 @@ STRATEGY_DIRECT = 0
 STRATEGY_GRAM_BF16 = 1
 STRATEGY_PROJECTION_BF16 = 2
 STRATEGY_GRAM_FP8 = 3
+STRATEGY_GRAM_INT8 = 4
 
 __all__ = [
     "METRIC_L1",
@@ -77,6 +78,8 @@ __all__ = [
     "STRATEGY_GRAM_BF16",
     "STRATEGY_PROJECTION_BF16",
     "STRATEGY_GRAM_FP8",
+    "STRATEGY_GRAM_INT8",
+    "int8_rows",
     "HipDenseL1ScanIndex",
     "HipExactIndex",
     "HipFlatL1AnnIndex",
@@ -369,6 +372,40 @@ class _DenseScan:
                 c_void_p(out.data_ptr()), out.stride(0), _torch_stream_ptr(stream),
             )
         )
+
+    def int8_estimates(self, device_queries, count: int, workspace: QueryWorkspace, out, tile: int = -1,
+                       stream=None, ld_queries=None):
+        """GRAM_INT8 estimates ||q||^2 + ||v||^2 - 2 s_q s_v (H + X / 256)
+        into the device tensor out (count x rows, row stride out.stride(0));
+        tile -1 runs the default block shape with its K split, 0..3 force a
+        shape (all give the same bits: the integer sums are exact)."""
+        ld = ld_queries if ld_queries is not None else self.dim
+        check(
+            _native.lib().uhann_scan_int8_estimates(
+                self._h, workspace.handle, c_void_p(device_queries.data_ptr()), ld, count,
+                c_void_p(out.data_ptr()), out.stride(0), tile, _torch_stream_ptr(stream),
+            )
+        )
+
+
+def int8_rows(device_rows, count: int, dim: int, stream=None, ld_rows=None):
+    """The GRAM_INT8 conversion of count device rows: (bytes, stats) device
+    tensors, bytes count x 128 ceil(dim / 64) uint8 (per 64 columns the high
+    plane, then the low plane), stats count x 32 uint8 records (float64
+    ||x||^2; float32 s, ||x - x^||, ||x^||, s ||b|| / 256, ||x||, pad)."""
+    import torch
+
+    ld = ld_rows if ld_rows is not None else dim
+    width = max(128, (dim + 63) // 64 * 128)
+    out = torch.zeros((max(count, 1), width), dtype=torch.uint8, device=device_rows.device)
+    stats = torch.zeros((max(count, 1), 32), dtype=torch.uint8, device=device_rows.device)
+    check(
+        _native.lib().uhann_int8_rows(
+            c_void_p(device_rows.data_ptr()), ld, count, dim, c_void_p(out.data_ptr()), c_void_p(stats.data_ptr()),
+            _torch_stream_ptr(stream),
+        )
+    )
+    return out[:count], stats[:count]
 
 
 class HipDenseL2ScanIndex(_DenseScan):

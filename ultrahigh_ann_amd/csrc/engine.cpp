@@ -386,7 +386,8 @@ int gram_tile()
 
 void Scan::enable_mixed(int strategy)
 {
-    if (strategy != UHANN_STRATEGY_GRAM_BF16 && strategy != UHANN_STRATEGY_GRAM_FP8) {
+    if (strategy != UHANN_STRATEGY_GRAM_BF16 && strategy != UHANN_STRATEGY_GRAM_FP8 &&
+        strategy != UHANN_STRATEGY_GRAM_INT8) {
         throw std::invalid_argument("unknown mixed strategy");
     }
     if (metric != UHANN_METRIC_L2) {
@@ -397,6 +398,24 @@ void Scan::enable_mixed(int strategy)
     // complete tiles without clamping row indices.
     const uint64_t mt = kern::gram_m_tile();
     const uint64_t padded_rows = (rows + mt - 1) / mt * mt;
+    if (strategy == UHANN_STRATEGY_GRAM_INT8) {
+        if (i8) return;
+        if (dim > kern::gram_i8_max_dim()) {
+            throw std::length_error("the int8 strategy keeps its sums exact for at most 65536 coordinates");
+        }
+        auto f = std::make_unique<I8Refs>();
+        const uint64_t step = kern::gram_i8_k_padding();
+        f->ldk = std::max<uint64_t>(step, (dim + 63) / 64 * step);
+        f->refs = DeviceBuffer<uint8_t>(device, checked_mul(padded_rows, f->ldk, "int8 references"));
+        check_hip(hipMemset(f->refs.data(), 0, f->refs.bytes()), "cannot clear the int8 references");
+        f->stats = DeviceBuffer<kern::I8Stats>(device, rows);
+        kern::to_i8x2_rows(refs.data(), ld, static_cast<uint32_t>(rows), static_cast<uint32_t>(dim), f->refs.data(),
+                           f->ldk, f->stats.data(), nullptr);
+        check_hip(hipDeviceSynchronize(), "int8 reference conversion failed");
+        payload_bytes += f->refs.bytes() + f->stats.bytes();
+        i8 = std::move(f);
+        return;
+    }
     if (strategy == UHANN_STRATEGY_GRAM_FP8) {
         if (fp8) return;
         auto f = std::make_unique<Fp8Refs>();
@@ -462,7 +481,21 @@ void Workspace::reserve_fp8(const Scan& scan)
     reserve_common(scan);
 }
 
-// Estimate matrix, counters and escalation buffers shared by both strategies.
+void Workspace::reserve_i8(const Scan& scan)
+{
+    if (scan.i8 == nullptr || qi8.bytes() != 0) return;
+    const uint64_t mt = kern::gram_m_tile();
+    const uint64_t mrows = (max_batch + mt - 1) / mt * mt;
+    qi8 = DeviceBuffer<uint8_t>(device, checked_mul(mrows, scan.i8->ldk, "int8 query workspace"));
+    qi8stats = DeviceBuffer<kern::I8Stats>(device, mrows);
+    payload_bytes += qi8.bytes() + qi8stats.bytes();
+    reserve_partials(kern::gram_i8_split_floats(static_cast<uint32_t>(max_batch), static_cast<uint32_t>(scan.rows),
+                                                static_cast<uint32_t>(scan.i8->ldk / kern::gram_i8_k_padding()),
+                                                device_cus(device)));
+    reserve_common(scan);
+}
+
+// Estimate matrix, counters and escalation buffers shared by the strategies.
 void Workspace::reserve_common(const Scan& scan)
 {
     if (approx.bytes() != 0) return;
@@ -598,6 +631,96 @@ void fp8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t 
     }
 }
 
+// The binary32 scan's relative error bound gamma_k (k = ceil(dim / 256) + 10:
+// the longest chain, the tree depth and the subtract and square of a term)
+// as the band factor (1 + gamma) / (1 - gamma).
+double fp32_band(uint64_t dim)
+{
+    const double k = std::ceil(static_cast<double>(dim) / 256.0) + 10.0;
+    const double u = std::ldexp(1.0, -24);
+    const double gamma = k * u / (1.0 - k * u);
+    return (1.0 + gamma) / (1.0 - gamma);
+}
+
+void i8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, uint64_t count,
+               uint64_t* out_idx, float* out_dist, hipStream_t stream)
+{
+    if (scan.i8 == nullptr) throw std::invalid_argument("int8 strategy not enabled for this index");
+    if (out_idx == nullptr || out_dist == nullptr) {
+        throw std::invalid_argument("the int8 strategy needs index and distance outputs");
+    }
+    ws.reserve_i8(scan);
+    const I8Refs& f = *scan.i8;
+    const uint32_t n = static_cast<uint32_t>(count);
+    ws.mark(0, stream);
+    kern::to_i8x2_rows(queries, ld_q, n, static_cast<uint32_t>(scan.dim), ws.qi8.data(), f.ldk, ws.qi8stats.data(),
+                       stream);
+    ws.mark(1, stream);
+    kern::gram_i8x2(ws.qi8.data(), f.ldk, n, f.refs.data(), f.ldk, static_cast<uint32_t>(scan.rows),
+                    static_cast<uint32_t>(f.ldk), ws.qi8stats.data(), f.stats.data(), ws.approx.data(), scan.rows,
+                    -1, ws.gram_partials.data(), ws.gram_partials.bytes() / sizeof(float), stream);
+    ws.mark(2, stream);
+    kern::zero_u32(ws.counters.data(), 8, stream);
+    kern::SelectArgs a;
+    a.nq = n;
+    a.rows = static_cast<uint32_t>(scan.rows);
+    a.dim = static_cast<uint32_t>(scan.dim);
+    a.approx = ws.approx.data();
+    a.ld_approx = scan.rows;
+    a.qstats8 = ws.qi8stats.data();
+    a.vstats8 = f.stats.data();
+    a.queries = queries;
+    a.ld_q = ld_q;
+    a.refs = scan.refs.data();
+    a.ld_ref = scan.ld;
+    a.fp32_band = fp32_band(scan.dim);
+    a.out_idx = out_idx;
+    a.out_dist = out_dist;
+    a.escalate_count = ws.counters.data();
+    a.escalate_list = ws.escalate_list.data();
+    a.stats = reinterpret_cast<kern::MixedStats*>(ws.counters.data() + 4);
+    kern::select_rerank_i8(a, stream);
+    ws.mark(3, stream);
+    kern::ScanArgs esc;
+    esc.refs = scan.refs.data();
+    esc.ld_ref = scan.ld;
+    esc.rows = static_cast<uint32_t>(scan.rows);
+    esc.queries = queries;
+    esc.ld_q = ld_q;
+    esc.nq = n;
+    esc.dim = static_cast<uint32_t>(scan.dim);
+    esc.keys = ws.keys.data();
+    kern::escalate_scan(esc, ws.counters.data(), ws.escalate_list.data(), n, ws.esc_queries.data(), scan.ld,
+                        out_idx, out_dist, stream);
+    ws.mark(4, stream);
+    if (ws.timing) {
+        check_hip(hipEventSynchronize(ws.events[4]), "HIP event wait failed");
+        for (int i = 0; i < Workspace::kStages; ++i) {
+            check_hip(hipEventElapsedTime(&ws.stage_ms[i], ws.events[i], ws.events[i + 1]),
+                      "cannot read HIP event time");
+        }
+    }
+}
+
+// Runs one batch with `strategy` on `stream`.
+void run_strategy(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, bool padded,
+                  uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream,
+                  int strategy)
+{
+    if (strategy == UHANN_STRATEGY_DIRECT) {
+        scan_launch(scan, queries, ld_q, count, ws.keys.data(), out_idx, out_dist, stream, padded,
+                    nullptr, &ws);
+    } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
+        mixed_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
+    } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {
+        fp8_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
+    } else if (strategy == UHANN_STRATEGY_GRAM_INT8) {
+        i8_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
+    } else {
+        throw std::invalid_argument("unknown query strategy");
+    }
+}
+
 }  // namespace uhann
 
 using namespace uhann;
@@ -613,23 +736,6 @@ uhann_workspace* checked_ws(const void* owner, uhann_workspace* ws, uint64_t cou
         throw std::invalid_argument("query count exceeds workspace capacity");
     }
     return ws;
-}
-
-// Runs one batch with `strategy` on the workspace stream/outputs.
-void run_strategy(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, bool padded,
-                  uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream,
-                  int strategy)
-{
-    if (strategy == UHANN_STRATEGY_DIRECT) {
-        scan_launch(scan, queries, ld_q, count, ws.keys.data(), out_idx, out_dist, stream, padded,
-                    nullptr, &ws);
-    } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
-        mixed_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
-    } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {
-        fp8_launch(scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
-    } else {
-        throw std::invalid_argument("unknown query strategy");
-    }
 }
 
 void copy_results(Workspace& ws, uint64_t count, uint64_t* out_idx, float* out_dist)
@@ -875,6 +981,51 @@ int uhann_scan_fp8_estimates(const uhann_scan* scan, uhann_workspace* ws, const 
         kern::gram_fp8(ws->q8.data(), f.ldk8, n, f.refs8.data(), f.ldk8, static_cast<uint32_t>(scan->rows),
                        static_cast<uint32_t>(f.ldk8), ws->q8stats.data(), f.stats.data(), device_out, ld_out,
                        ws->gram_partials.data(), ws->gram_partials.bytes() / sizeof(float), s);
+    });
+}
+
+int uhann_scan_int8_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,
+                              uint64_t ld_queries, uint64_t count, float* device_out, uint64_t ld_out, int tile,
+                              void* stream)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        checked_ws(scan, ws, count);
+        if (scan->i8 == nullptr) throw std::invalid_argument("int8 strategy not enabled for this index");
+        if (tile < -1 || tile >= kern::gram_i8_tile_count()) throw std::invalid_argument("unknown int8 Gram tile");
+        if (count == 0) return;
+        if (device_queries == nullptr || device_out == nullptr) {
+            throw std::invalid_argument("device pointers must not be null");
+        }
+        if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
+        if (ld_out < scan->rows) throw std::invalid_argument("output row stride is below the reference count");
+        select_device(scan->device);
+        ws->reserve_i8(*scan);
+        const I8Refs& f = *scan->i8;
+        const auto s = static_cast<hipStream_t>(stream);
+        const uint32_t n = static_cast<uint32_t>(count);
+        kern::to_i8x2_rows(device_queries, ld_queries, n, static_cast<uint32_t>(scan->dim), ws->qi8.data(), f.ldk,
+                           ws->qi8stats.data(), s);
+        kern::gram_i8x2(ws->qi8.data(), f.ldk, n, f.refs.data(), f.ldk, static_cast<uint32_t>(scan->rows),
+                        static_cast<uint32_t>(f.ldk), ws->qi8stats.data(), f.stats.data(), device_out, ld_out, tile,
+                        ws->gram_partials.data(), ws->gram_partials.bytes() / sizeof(float), s);
+    });
+}
+
+int uhann_int8_rows(const float* device_rows, uint64_t ld_rows, uint64_t count, uint64_t dim, uint8_t* device_bytes,
+                    void* device_stats, void* stream)
+{
+    return guarded([&] {
+        if (count == 0) return;
+        if (device_rows == nullptr || device_bytes == nullptr || device_stats == nullptr) {
+            throw std::invalid_argument("device pointers must not be null");
+        }
+        if (ld_rows < dim) throw std::invalid_argument("row stride is below dim");
+        if (count > 0xffffffffULL || dim > kern::gram_i8_max_dim()) throw std::length_error("too many rows or columns");
+        const uint64_t ld = std::max<uint64_t>(kern::gram_i8_k_padding(), (dim + 63) / 64 * kern::gram_i8_k_padding());
+        kern::to_i8x2_rows(device_rows, ld_rows, static_cast<uint32_t>(count), static_cast<uint32_t>(dim),
+                           device_bytes, ld, static_cast<kern::I8Stats*>(device_stats),
+                           static_cast<hipStream_t>(stream));
     });
 }
 

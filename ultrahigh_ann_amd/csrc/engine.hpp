@@ -36,6 +36,15 @@ struct Fp8Refs {
     DeviceBuffer<kern::RowStats> stats;
 };
 
+// 16-bit fixed-point copy of the references for GRAM_INT8: two int8 planes
+// interleaved per 64 columns (ldk8x2 = 128 ceil(dim / 64) bytes per row),
+// rows padded to whole Gram tiles, plus row statistics.
+struct I8Refs {
+    uint64_t ldk = 0;
+    DeviceBuffer<uint8_t> refs;
+    DeviceBuffer<kern::I8Stats> stats;
+};
+
 // Dense FP32 references resident on one device, row-major with a leading
 // dimension padded to a multiple of 4 floats.
 struct Scan {
@@ -57,6 +66,7 @@ struct Scan {
     std::string device_name;
     std::unique_ptr<GramRefs> gram;  // set by enable_mixed(GRAM_BF16)
     std::unique_ptr<Fp8Refs> fp8;    // set by enable_mixed(GRAM_FP8)
+    std::unique_ptr<I8Refs> i8;      // set by enable_mixed(GRAM_INT8)
 
     // Autotuned direct-scan kernels per batch bucket b (batches of
     // 2^(b-1)+1 .. 2^b queries); empty entries fall back to the heuristic.
@@ -101,10 +111,13 @@ struct Workspace {
     DeviceBuffer<float> esc_queries;    // escalated queries, gathered on the device
     DeviceBuffer<uint8_t> q8;           // FP8 strategy: e4m3 queries
     DeviceBuffer<kern::RowStats> q8stats;
+    DeviceBuffer<uint8_t> qi8;          // INT8 strategy: fixed-point queries
+    DeviceBuffer<kern::I8Stats> qi8stats;
     DeviceBuffer<float> gram_partials;  // K-split partial sums of the Gram tail tiles
     void reserve_partials(uint64_t floats);
     void reserve_mixed(const Scan& scan);
     void reserve_fp8(const Scan& scan);
+    void reserve_i8(const Scan& scan);
     void reserve_common(const Scan& scan);
 
     // Optional stage timing (HIP events on the launch stream):
@@ -136,8 +149,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
                  bool queries_padded, const kern::ScanTile* forced_tile, Workspace* timed = nullptr);
 
 // Mixed-precision L2 query (BF16 Gram -> candidate set -> exact re-rank ->
-// binary64 near ties -> direct scan for escalated queries). Synchronises
-// `stream` once to read the escalation count.
+// binary64 near ties -> direct scan for escalated queries, on the device).
 void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q,
                   uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream);
 
@@ -145,6 +157,16 @@ void mixed_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_
 // binary64 near ties); no host synchronisation.
 void fp8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, uint64_t count,
                 uint64_t* out_idx, float* out_dist, hipStream_t stream);
+
+// Certified INT8 query (fixed-point int8 Gram with exact int32 sums ->
+// deterministic candidate set -> exact re-rank -> binary64 near ties ->
+// direct scan for escalated queries, on the device).
+void i8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, uint64_t count,
+               uint64_t* out_idx, float* out_dist, hipStream_t stream);
+
+// Any strategy's batch on `stream` (DIRECT, GRAM_BF16, GRAM_FP8, GRAM_INT8).
+void run_strategy(const Scan& scan, Workspace& ws, const float* queries, uint64_t ld_q, bool padded,
+                  uint64_t count, uint64_t* out_idx, float* out_dist, hipStream_t stream, int strategy);
 
 }  // namespace uhann
 

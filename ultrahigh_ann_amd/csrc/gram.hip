@@ -1182,6 +1182,462 @@ select_topk_rerank_kernel(SelectArgs a)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// GRAM_INT8: rows as 16-bit fixed point in two int8 planes; inner products
+// summed exactly in int32 on v_mfma_i32_32x32x32_i8.
+//
+// x^ = s (a + b / 256), so <q^, v^> = s_q s_v (H + X / 256 + L / 65536) with
+// H = <a_q, a_v>, X = <a_q, b_v> + <b_q, a_v> and L = <b_q, b_v>. H and X are
+// integer sums (|H| <= 127^2 dim, |X| <= 2 127 128 dim < 2^31 for dim <=
+// 65536), so the MFMAs add them without rounding; L is left out and bounded
+// by |s_q s_v L / 65536| <= lo_q lo_v (Cauchy-Schwarz). With q = q^ + e_q,
+// v = v^ + e_v: <q, v> - <q^, v^> = <q, e_v> + <e_q, v^>, bounded by
+// ||q|| ||e_v|| + ||e_q|| ||v^||. Every norm is of the row's actual values
+// (binary64 sums, rounded up), so the bound holds for every input: there is
+// no model of the rounding errors.
+
+typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef int intx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBKI8 = 128;               // bytes per row and K step: 64 columns of a, then of b
+constexpr int kLdsRowI8 = kBKI8 + 16;    // LDS row stride in bytes (padding)
+constexpr uint32_t kI8MaxDim = 65536;    // 2 * 127 * 128 * 65536 < 2^31
+constexpr int kI8Code = 32512;           // |n| <= 127 * 256
+
+// A float no smaller than v >= 0 (binary32 rounding moves a value by at most
+// 2^-24 of itself, less than the 2^-20 added first).
+__device__ __forceinline__ float round_up_f(double v) { return static_cast<float>(v * (1.0 + 0x1p-20)); }
+
+// One block of 256 threads per row: max |x|, then codes, bytes and sums.
+__global__ void __launch_bounds__(256)
+to_i8x2_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows, uint32_t dim,
+                    uint8_t* __restrict__ dst, uint64_t ld_dst, I8Stats* __restrict__ stats)
+{
+    __shared__ double scratch[8];
+    __shared__ float fscratch[8];
+    const uint32_t row = blockIdx.x;
+    if (row >= rows) return;
+    const float* s = src + static_cast<uint64_t>(row) * ld_src;
+    uint8_t* o = dst + static_cast<uint64_t>(row) * ld_dst;
+    const bool vec = ld_src % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15U) == 0;
+    const uint32_t full = vec ? dim / 4 * 4 : 0;
+    float mx = 0.0F;
+    int bad = 0;
+    {
+        auto see = [&](float x) {
+            mx = fmaxf(mx, fabsf(x));
+            bad |= !isfinite(x);
+        };
+        const uint32_t step = blockDim.x * 4;
+        uint32_t c = threadIdx.x * 4;
+        for (; c + 3 * step < full; c += 4 * step) {
+            float4 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const float4*>(s + c + u * step);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                see(x[u].x);
+                see(x[u].y);
+                see(x[u].z);
+                see(x[u].w);
+            }
+        }
+        for (; c < full; c += step) {
+            const float4 x = *reinterpret_cast<const float4*>(s + c);
+            see(x.x);
+            see(x.y);
+            see(x.z);
+            see(x.w);
+        }
+        for (uint32_t e = full + threadIdx.x; e < dim; e += blockDim.x) see(s[e]);
+    }
+    mx = block_max_f32(mx, fscratch);
+    const bool finite = __syncthreads_or(bad) == 0;
+    const float scale = !finite ? __builtin_nanf("") : (mx > 0.0F ? mx / 127.0F : 1.0F);
+    const float inv = (finite && mx > 0.0F) ? static_cast<float>(kI8Code) / mx : 0.0F;
+    const double sd = static_cast<double>(scale) * 0.00390625;  // s / 256, exact
+    double n2 = 0.0, e2 = 0.0, h2 = 0.0, b2 = 0.0;
+    // Columns in groups of four (padding columns are zero): four a bytes and
+    // four b bytes per group, written as two 32-bit words.
+    const uint32_t pdim = static_cast<uint32_t>(ld_dst / 2);
+    for (uint32_t c = threadIdx.x * 4; c < pdim; c += blockDim.x * 4) {
+        float x[4];
+        if (vec && c + 3 < dim) {
+            const float4 v = *reinterpret_cast<const float4*>(s + c);
+            x[0] = v.x;
+            x[1] = v.y;
+            x[2] = v.z;
+            x[3] = v.w;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = c + u < dim ? s[c + u] : 0.0F;
+        }
+        uint32_t pa = 0, pb = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            int n = 0;
+            if (finite) {
+                n = static_cast<int>(rintf(x[u] * inv));
+                n = n < -kI8Code ? -kI8Code : (n > kI8Code ? kI8Code : n);
+            }
+            const int a = (n + 128) >> 8;  // floor((n + 128) / 256): a in [-127, 127]
+            const int b = n - a * 256;     // b in [-128, 127]
+            const double xd = x[u];
+            const double hd = sd * static_cast<double>(n);  // exact: 24-bit s times 16-bit n
+            const double ed = xd - hd;
+            n2 += xd * xd;
+            e2 += ed * ed;
+            h2 += hd * hd;
+            b2 += static_cast<double>(b * b);
+            pa |= (static_cast<uint32_t>(a) & 255U) << (8 * u);
+            pb |= (static_cast<uint32_t>(b) & 255U) << (8 * u);
+        }
+        uint8_t* chunk = o + (c / 64U) * 128U + (c % 64U);
+        *reinterpret_cast<uint32_t*>(chunk) = pa;
+        *reinterpret_cast<uint32_t*>(chunk + 64) = pb;
+    }
+    n2 = block_sum_f64(n2, scratch);
+    e2 = block_sum_f64(e2, scratch);
+    h2 = block_sum_f64(h2, scratch);
+    b2 = block_sum_f64(b2, scratch);
+    if (threadIdx.x == 0) {
+        I8Stats st{};
+        if (finite) {
+            st.norm2 = n2;
+            st.scale = scale;
+            st.err = round_up_f(sqrt(e2));
+            st.hat = round_up_f(sqrt(h2));
+            st.lo = round_up_f(sd * sqrt(b2));
+            st.norm = round_up_f(sqrt(n2));
+        } else {
+            const float nan = __builtin_nanf("");
+            st.norm2 = static_cast<double>(nan);
+            st.scale = st.err = st.hat = st.lo = st.norm = nan;
+        }
+        stats[row] = st;
+    }
+}
+
+// est = ||q||^2 + ||v||^2 - 2 s_q s_v (H + X / 256): H + X / 256 and s_q s_v
+// are exact in binary64, their product and the sums round once each, and the
+// result is rounded to binary32.
+__device__ __forceinline__ float i8_estimate(int h, int x, const I8Stats& q, double nv, double sv)
+{
+    const double t = static_cast<double>(h) + static_cast<double>(x) * 0.00390625;
+    const double dot = (static_cast<double>(q.scale) * sv) * t;
+    return static_cast<float>((q.norm2 + nv) - 2.0 * dot);
+}
+
+template <int TM, int TN>
+__device__ __forceinline__ void i8_epilogue(const intx16 (&h)[TM][TN], const intx16 (&x)[TM][TN], uint32_t mw,
+                                            uint32_t nw, int lane, uint32_t M, uint32_t N,
+                                            const I8Stats* __restrict__ qs, const I8Stats* __restrict__ vs,
+                                            float* __restrict__ out, uint64_t ldo)
+{
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const uint32_t n = nw + j * 32 + (lane % 32);
+            if (n >= N) continue;
+            const double nv = vs[n].norm2;
+            const double sv = vs[n].scale;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const uint32_t m = mw + i * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (m < M) out[static_cast<uint64_t>(m) * ldo + n] = i8_estimate(h[i][j][v], x[i][j][v], qs[m], nv, sv);
+            }
+        }
+    }
+}
+
+// A split tile's two int32 accumulator sets (bit patterns in the float
+// scratch): set s, chunk c = (i*TN + j)*4 + v/4 at ((s*TM*TN*4 + c)*NT + tid)*4.
+template <int NT, int TM, int TN>
+__device__ __forceinline__ void store_i8_partials(float* __restrict__ dst, const intx16 (&h)[TM][TN],
+                                                  const intx16 (&x)[TM][TN], int tid)
+{
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int chunk = (i * TN + j) * 4 + c;
+                *reinterpret_cast<intx4*>(dst + (static_cast<uint64_t>(chunk) * NT + tid) * 4) =
+                    intx4{h[i][j][4 * c], h[i][j][4 * c + 1], h[i][j][4 * c + 2], h[i][j][4 * c + 3]};
+                *reinterpret_cast<intx4*>(dst + (static_cast<uint64_t>(TM * TN * 4 + chunk) * NT + tid) * 4) =
+                    intx4{x[i][j][4 * c], x[i][j][4 * c + 1], x[i][j][4 * c + 2], x[i][j][4 * c + 3]};
+            }
+}
+
+// Sums the K parts of one split tile (int32, exact) and applies the epilogue.
+template <int WM, int WN, int TM, int TN>
+__global__ void __launch_bounds__(64 * WM * WN)
+gram_i8_fixup_kernel(GramSplit sp, uint32_t M, uint32_t N, const I8Stats* __restrict__ qs,
+                     const I8Stats* __restrict__ vs, float* __restrict__ out, uint64_t ldo, uint32_t tiles_m,
+                     uint32_t tiles_n, uint32_t group_m)
+{
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BM = 32 * TM * WM;
+    constexpr int BN = 32 * TN * WN;
+    const int tid = threadIdx.x;
+    const int lane = tid % 64;
+    const int wave = tid / 64;
+    const uint2 o = tile_origin(sp.full + blockIdx.x, tiles_m, tiles_n, group_m);
+    intx16 h[TM][TN], x[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) h[i][j][v] = x[i][j][v] = 0;
+    for (uint32_t p = 0; p < sp.parts; ++p) {
+        const float* src = sp.partials + (static_cast<uint64_t>(blockIdx.x) * sp.parts + p) * (2 * BM * BN);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int chunk = (i * TN + j) * 4 + c;
+                    const intx4 hv = *reinterpret_cast<const intx4*>(src + (static_cast<uint64_t>(chunk) * NT + tid) * 4);
+                    const intx4 xv = *reinterpret_cast<const intx4*>(
+                        src + (static_cast<uint64_t>(TM * TN * 4 + chunk) * NT + tid) * 4);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        h[i][j][4 * c + e] += hv[e];
+                        x[i][j][4 * c + e] += xv[e];
+                    }
+                }
+    }
+    i8_epilogue<TM, TN>(h, x, o.x * BM + (wave / WN) * 32 * TM, o.y * BN + (wave % WN) * 32 * TN, lane, M, N, qs,
+                        vs, out, ldo);
+}
+
+// Blocks of WM x WN waves, each a (32 TM) x (32 TN) output tile with two
+// int32 accumulator sets (H and X) of TM x TN 32x32 MFMA tiles. A K step is
+// one 128-byte chunk per row (64 columns of a, then of b), double-buffered in
+// LDS with the next step's loads issued before the barrier, as in the BF16
+// kernel. Per 32 columns and output tile: one MFMA for H, two for X.
+template <int WM, int WN, int TM, int TN>
+__global__ void __launch_bounds__(64 * WM * WN)
+gram_i8x2_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const uint8_t* __restrict__ V,
+                 uint64_t ldv, uint32_t N, uint32_t K, const I8Stats* __restrict__ qs,
+                 const I8Stats* __restrict__ vs, float* __restrict__ out, uint64_t ldo, uint32_t tiles_m,
+                 uint32_t tiles_n, uint32_t group_m, GramSplit sp)
+{
+    constexpr int NT = 64 * WM * WN;
+    constexpr int BM = 32 * TM * WM;
+    constexpr int BN = 32 * TN * WN;
+    static_assert((BM * 8) % NT == 0 && (BN * 8) % NT == 0, "tile rows must split evenly over the threads");
+    constexpr int CA = BM * 8 / NT;  // 16-byte chunks per thread per K step
+    constexpr int CB = BN * 8 / NT;
+    __shared__ __attribute__((aligned(16))) uint8_t As[2 * BM * kLdsRowI8];
+    __shared__ __attribute__((aligned(16))) uint8_t Bs[2 * BN * kLdsRowI8];
+
+    const GramBlock gb = gram_block(blockIdx.x, sp, K, kBKI8);
+    const uint2 o = tile_origin(gb.tile, tiles_m, tiles_n, group_m);
+    const uint32_t m0 = o.x * BM;
+    const uint32_t n0 = o.y * BN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid % 64;
+    const int wave = tid / 64;
+    const int wm = (wave / WN) * 32 * TM;
+    const int wn = (wave % WN) * 32 * TN;
+
+    u32x4 ra[CA], rb[CB];
+    const uint8_t* qbase = Q + static_cast<uint64_t>(m0 + tid / 8) * ldq + (tid % 8) * 16;
+    const uint8_t* vbase = V + static_cast<uint64_t>(n0 + tid / 8) * ldv + (tid % 8) * 16;
+    const uint64_t qstep = static_cast<uint64_t>(NT / 8) * ldq;
+    const uint64_t vstep = static_cast<uint64_t>(NT / 8) * ldv;
+    const int lds0 = (tid / 8) * kLdsRowI8 + (tid % 8) * 16;
+    constexpr int kLdsStep = (NT / 8) * kLdsRowI8;
+    // MFMA operand fragments: lane holds row (lane % 32) and 16 K values
+    // 16*(lane/32) .. +15 of each 32-column step, the same assignment for
+    // both operands, as the dot product needs.
+    const int a_off = (wm + lane % 32) * kLdsRowI8 + (lane / 32) * 16;
+    const int b_off = (wn + lane % 32) * kLdsRowI8 + (lane / 32) * 16;
+
+    intx16 acc_h[TM][TN], acc_x[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc_h[i][j][v] = acc_x[i][j][v] = 0;
+
+#define UHANN_I8_LOAD(k0)                                                                      \
+    _Pragma("unroll") for (int c = 0; c < CA; ++c) ra[c] =                                     \
+        *reinterpret_cast<const u32x4*>(qbase + c * qstep + (k0));                             \
+    _Pragma("unroll") for (int c = 0; c < CB; ++c) rb[c] =                                     \
+        *reinterpret_cast<const u32x4*>(vbase + c * vstep + (k0));                             \
+    __builtin_amdgcn_sched_barrier(0);
+#define UHANN_I8_STORE(buf)                                                                    \
+    _Pragma("unroll") for (int c = 0; c < CA; ++c)                                             \
+        *reinterpret_cast<u32x4*>(As + (buf) * BM * kLdsRowI8 + lds0 + c * kLdsStep) = ra[c];  \
+    _Pragma("unroll") for (int c = 0; c < CB; ++c)                                             \
+        *reinterpret_cast<u32x4*>(Bs + (buf) * BN * kLdsRowI8 + lds0 + c * kLdsStep) = rb[c];
+#define UHANN_I8_COMPUTE(buf)                                                                  \
+    _Pragma("unroll") for (int kk = 0; kk < 64; kk += 32)                                      \
+    {                                                                                          \
+        const uint8_t* ab = As + (buf) * BM * kLdsRowI8 + a_off + kk;                          \
+        const uint8_t* bb = Bs + (buf) * BN * kLdsRowI8 + b_off + kk;                          \
+        intx4 ah[TM], al[TM], bh[TN], bl[TN];                                                  \
+        _Pragma("unroll") for (int i = 0; i < TM; ++i) {                                       \
+            ah[i] = *reinterpret_cast<const intx4*>(ab + i * 32 * kLdsRowI8);                  \
+            al[i] = *reinterpret_cast<const intx4*>(ab + i * 32 * kLdsRowI8 + 64);             \
+        }                                                                                      \
+        _Pragma("unroll") for (int j = 0; j < TN; ++j) {                                       \
+            bh[j] = *reinterpret_cast<const intx4*>(bb + j * 32 * kLdsRowI8);                  \
+            bl[j] = *reinterpret_cast<const intx4*>(bb + j * 32 * kLdsRowI8 + 64);             \
+        }                                                                                      \
+        _Pragma("unroll") for (int i = 0; i < TM; ++i)                                         \
+            _Pragma("unroll") for (int j = 0; j < TN; ++j) acc_h[i][j] =                       \
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[i], bh[j], acc_h[i][j], 0, 0, 0);     \
+        _Pragma("unroll") for (int i = 0; i < TM; ++i)                                         \
+            _Pragma("unroll") for (int j = 0; j < TN; ++j) acc_x[i][j] =                       \
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[i], bl[j], acc_x[i][j], 0, 0, 0);     \
+        _Pragma("unroll") for (int i = 0; i < TM; ++i)                                         \
+            _Pragma("unroll") for (int j = 0; j < TN; ++j) acc_x[i][j] =                       \
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(al[i], bh[j], acc_x[i][j], 0, 0, 0);     \
+    }
+
+    const uint32_t k_last = gb.k_end - kBKI8;
+    UHANN_I8_LOAD(gb.k_begin)
+    UHANN_I8_STORE(0)
+    UHANN_I8_LOAD(min(gb.k_begin + kBKI8, k_last))
+    __syncthreads();
+    int cur = 0;
+    for (uint32_t k0 = gb.k_begin + kBKI8; k0 < gb.k_end; k0 += kBKI8) {
+        UHANN_I8_COMPUTE(cur)
+        UHANN_I8_STORE(cur ^ 1)
+        UHANN_I8_LOAD(min(k0 + kBKI8, k_last))
+        __syncthreads();
+        cur ^= 1;
+    }
+    UHANN_I8_COMPUTE(cur)
+#undef UHANN_I8_LOAD
+#undef UHANN_I8_STORE
+#undef UHANN_I8_COMPUTE
+
+    if (gb.part >= 0) {
+        const uint64_t unit = static_cast<uint64_t>(gb.tile - sp.full) * sp.parts + gb.part;
+        store_i8_partials<NT, TM, TN>(sp.partials + unit * (2 * BM * BN), acc_h, acc_x, tid);
+        return;
+    }
+    i8_epilogue<TM, TN>(acc_h, acc_x, m0 + wm, n0 + wn, lane, M, N, qs, vs, out, ldo);
+}
+
+// Lower and upper ends of ||q - v||^2 from a GRAM_INT8 estimate e: the bound
+// of the file comment above, computed in binary64 from the rounded-up row
+// norms, plus 2^-23 |e| for the binary32 rounding of the estimate and
+// 1e-10 (||q||^2 + ||v||^2) for the binary64 sums and roundings behind it;
+// the ends are rounded outwards to binary32. A non-finite estimate gives NaN
+// ends (never a candidate; the query escalates when nothing else is).
+__device__ __forceinline__ void i8_ends(const I8Stats& q, const I8Stats& v, float e, float& lo, float& up)
+{
+    const double beta = static_cast<double>(q.norm) * v.err + static_cast<double>(q.err) * v.hat +
+                        static_cast<double>(q.lo) * v.lo;
+    const double tau = 2.0 * beta + 0x1p-23 * fabs(static_cast<double>(e)) + 1e-10 * (q.norm2 + v.norm2);
+    lo = __double2float_rd(static_cast<double>(e) - tau);
+    up = __double2float_ru(static_cast<double>(e) + tau);
+}
+
+// One block per query. U = min_r (upper end); every row whose binary32
+// distance could be the binary32 minimum m, or lie in its near-tie band
+// (<= m (1+g)/(1-g)), has ||q - v||^2 <= U ((1+g)/(1-g))^2 and is kept. The
+// candidates are re-ranked exactly as in the BF16 strategy; more than
+// kMaxCandidates (or none: a non-finite query) escalates to the direct scan.
+__global__ void __launch_bounds__(kSelThreads)
+select_rerank_i8_kernel(SelectArgs a)
+{
+    __shared__ uint32_t s_min;
+    __shared__ uint32_t s_count;
+    __shared__ uint32_t s_cand[kMaxCandidates];
+    __shared__ float s_dist[kMaxCandidates];
+    __shared__ float s_group[8][kRerankRows];
+    __shared__ double s_d64[kMaxCandidates];
+    __shared__ uint32_t s_best;
+    __shared__ float s_best_d;
+
+    const uint32_t q = blockIdx.x;
+    if (q >= a.nq) return;
+    const int t = threadIdx.x;
+    const I8Stats qs = a.qstats8[q];
+    const float* arow = a.approx + static_cast<uint64_t>(q) * a.ld_approx;
+
+    if (t == 0) {
+        s_min = 0xffffffffU;
+        s_count = 0;
+    }
+    __syncthreads();
+    uint32_t local = 0xffffffffU;
+    float lo_keep[kKeepLo];
+#pragma unroll
+    for (int i = 0; i < kKeepLo; ++i) {
+        const uint32_t r = t + static_cast<uint32_t>(i) * kSelThreads;
+        lo_keep[i] = __builtin_inff();
+        if (r < a.rows) {
+            float lo, up;
+            i8_ends(qs, a.vstats8[r], arow[r], lo, up);
+            local = min(local, ordered(up));
+            lo_keep[i] = lo;
+        }
+    }
+    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
+        float lo, up;
+        i8_ends(qs, a.vstats8[r], arow[r], lo, up);
+        local = min(local, ordered(up));
+    }
+    for (int off = 32; off >= 1; off /= 2) local = min(local, __shfl_down(local, off));
+    if (t % 64 == 0) atomicMin(&s_min, local);
+    __syncthreads();
+    // Rounded up: the product's rounding is far below the 2^-40 added.
+    const double threshold = static_cast<double>(unordered(s_min)) * a.fp32_band * a.fp32_band * (1.0 + 0x1p-40);
+#pragma unroll
+    for (int i = 0; i < kKeepLo; ++i) {
+        if (static_cast<double>(lo_keep[i]) <= threshold) {
+            const uint32_t slot = atomicAdd(&s_count, 1U);
+            if (slot < kMaxCandidates) s_cand[slot] = t + static_cast<uint32_t>(i) * kSelThreads;
+        }
+    }
+    for (uint32_t r = t + kKeepLo * kSelThreads; r < a.rows; r += kSelThreads) {
+        float lo, up;
+        i8_ends(qs, a.vstats8[r], arow[r], lo, up);
+        if (static_cast<double>(lo) <= threshold) {
+            const uint32_t slot = atomicAdd(&s_count, 1U);
+            if (slot < kMaxCandidates) s_cand[slot] = r;
+        }
+    }
+    __syncthreads();
+    const uint32_t count = s_count;
+    if (count > kMaxCandidates || count == 0) {
+        if (t == 0) {
+            const uint32_t slot = atomicAdd(a.escalate_count, 1U);
+            a.escalate_list[slot] = q;
+            a.out_idx[q] = 0;
+            a.out_dist[q] = 0.0F;
+            if (a.stats != nullptr) atomicAdd(&a.stats->escalated, 1U);
+        }
+        return;
+    }
+
+    uint32_t best;
+    float best_d;
+    rerank_pick<kRerankRows>(a, q, s_cand, count, s_dist, s_group, s_d64, &s_best, &s_best_d, best, best_d);
+    if (t == 0) {
+        a.out_idx[q] = best;
+        a.out_dist[q] = best_d;
+        if (a.stats != nullptr) {
+            atomicAdd(&a.stats->candidates, count);
+            atomicAdd(&a.stats->certified, 1U);
+        }
+    }
+}
+
 }  // namespace
 
 void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, __bf16* dst,
@@ -1361,5 +1817,96 @@ uint32_t gram_fp8_k_padding() { return kBK8; }
 uint32_t gram_k_padding() { return kBK; }
 int gram_tile_count() { return 8; }
 uint32_t gram_m_tile() { return 256; }
+
+void to_i8x2_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, uint8_t* dst, uint64_t ld_dst,
+                  I8Stats* stats, hipStream_t stream)
+{
+    if (rows == 0) return;
+    if (ld_dst % kBKI8 != 0 || ld_dst / 2 < dim) throw std::invalid_argument("int8 rows need 128 ceil(dim / 64) bytes");
+    to_i8x2_rows_kernel<<<rows, 256, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    check_launch("int8 conversion kernel");
+}
+
+namespace {
+
+// GRAM_INT8 block shapes: waves WM x WN, MFMA tiles per wave TM x TN.
+//   0: 4x2 waves, 2x2 tiles (256 x 128, 8 waves: 128 accumulator registers per lane)
+//   1: 4x4 waves, 2x1 tiles (256 x 128, 16 waves)
+//   2: 2x4 waves, 2x2 tiles (128 x 256, 8 waves)
+//   3: 4x4 waves, 1x2 tiles (128 x 256, 16 waves)
+struct I8Shape {
+    int wm, wn, tm, tn;
+};
+constexpr I8Shape kI8Shapes[4] = {{4, 2, 2, 2}, {4, 4, 2, 1}, {2, 4, 2, 2}, {4, 4, 1, 2}};
+constexpr int kI8DefaultTile = 0;
+
+}  // namespace
+
+uint64_t gram_i8_split_floats(uint32_t max_m, uint32_t n, uint32_t k_steps, uint32_t cus)
+{
+    const I8Shape sh = kI8Shapes[kI8DefaultTile];
+    const uint32_t bm = 32U * sh.tm * sh.wm;
+    const uint32_t bn = 32U * sh.tn * sh.wn;
+    const uint32_t tiles_n = (n + bn - 1) / bn;
+    uint64_t need = 0;
+    for (uint32_t tm = 1; tm <= (max_m + bm - 1) / bm; ++tm) {
+        const GramSplit sp = gram_split_plan(tm * tiles_n, k_steps, cus);
+        need = std::max<uint64_t>(need, static_cast<uint64_t>(sp.tail) * sp.parts * 2U * bm * bn);
+    }
+    return need;
+}
+
+void gram_i8x2(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uint64_t ldv, uint32_t n, uint32_t k,
+               const I8Stats* qs, const I8Stats* vs, float* out, uint64_t ldo, int tile, float* partials,
+               uint64_t partial_floats, hipStream_t stream)
+{
+    if (m == 0 || n == 0) return;
+    if (k % kBKI8 != 0) throw std::invalid_argument("int8 Gram K must be a multiple of 128 bytes");
+    if (k / 2 > kI8MaxDim) throw std::length_error("int8 Gram rows are limited to 65536 columns");
+    if (ldq % 16 != 0 || ldv % 16 != 0) throw std::invalid_argument("int8 Gram rows must be 16-byte aligned");
+    if (tile < -1 || tile >= gram_i8_tile_count()) throw std::invalid_argument("unknown int8 Gram tile");
+    const I8Shape sh = kI8Shapes[tile < 0 ? kI8DefaultTile : tile];
+    const uint32_t bm = 32U * sh.tm * sh.wm;
+    const uint32_t bn = 32U * sh.tn * sh.wn;
+    const uint32_t tiles_m = (m + bm - 1) / bm;
+    const uint32_t tiles_n = (n + bn - 1) / bn;
+    const uint32_t group_m = 8;
+    GramSplit sp;
+    sp.full = tiles_m * tiles_n;
+    if (tile < 0) sp = split_for(tiles_m * tiles_n, k / kBKI8, static_cast<uint64_t>(2U * bm * bn), partials, partial_floats);
+    const dim3 grid(sp.full + sp.tail * sp.parts);
+    const dim3 block(64 * sh.wm * sh.wn);
+#define UHANN_I8_CASE(T, A, B, C, D)                                                                       \
+    case T:                                                                                                \
+        gram_i8x2_kernel<A, B, C, D><<<grid, block, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo,  \
+                                                                 tiles_m, tiles_n, group_m, sp);           \
+        if (sp.tail > 0) {                                                                                 \
+            gram_i8_fixup_kernel<A, B, C, D><<<sp.tail, block, 0, stream>>>(sp, m, n, qs, vs, out, ldo,       \
+                                                                            tiles_m, tiles_n, group_m);    \
+        }                                                                                                  \
+        break;
+    switch (tile < 0 ? kI8DefaultTile : tile) {
+        UHANN_I8_CASE(0, 4, 2, 2, 2)
+        UHANN_I8_CASE(1, 4, 4, 2, 1)
+        UHANN_I8_CASE(2, 2, 4, 2, 2)
+        UHANN_I8_CASE(3, 4, 4, 1, 2)
+    default:
+        break;
+    }
+#undef UHANN_I8_CASE
+    check_launch("int8 Gram kernel");
+}
+
+void select_rerank_i8(const SelectArgs& args, hipStream_t stream)
+{
+    if (args.nq == 0) return;
+    if (args.qstats8 == nullptr || args.vstats8 == nullptr) throw std::invalid_argument("int8 selection needs row statistics");
+    select_rerank_i8_kernel<<<args.nq, kSelThreads, 0, stream>>>(args);
+    check_launch("int8 candidate selection / re-rank kernel");
+}
+
+uint32_t gram_i8_k_padding() { return kBKI8; }
+uint32_t gram_i8_max_dim() { return kI8MaxDim; }
+int gram_i8_tile_count() { return 4; }
 
 }  // namespace uhann::kern

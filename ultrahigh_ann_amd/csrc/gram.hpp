@@ -21,6 +21,20 @@ struct RowStats {
     float pad[2];
 };
 
+// GRAM_INT8 per-row statistics. A row x is held as 16-bit fixed point
+// x^ = s (a + b / 256) with int8 planes a (in [-127, 127]) and b (in
+// [-128, 127]) and s = max|x| / 127; every norm is of the actual values and
+// rounded up, so the error bound built from them is deterministic.
+struct I8Stats {
+    double norm2;  // ||x||^2 of the binary32 row (binary64 sum)
+    float scale;   // s (1 for a zero row, NaN for a non-finite row)
+    float err;     // ||x - x^||_2
+    float hat;     // ||x^||_2
+    float lo;      // s ||b||_2 / 256
+    float norm;    // ||x||_2
+    float pad;
+};
+
 struct MixedStats {
     unsigned int candidates;     // sum over queries of the candidate-set size
     unsigned int fp64_reranked;  // queries decided in binary64
@@ -59,6 +73,8 @@ struct SelectArgs {
     const float* refs = nullptr;    // binary32 references, row stride ld_ref
     uint64_t ld_ref = 0;
     double fp32_band = 1.0;         // (1+gamma)/(1-gamma) of the binary32 scan
+    const I8Stats* qstats8 = nullptr;  // GRAM_INT8 row statistics (instead of qstats / vstats)
+    const I8Stats* vstats8 = nullptr;
     uint64_t* out_idx = nullptr;
     float* out_dist = nullptr;
     unsigned int* escalate_count = nullptr;
@@ -105,6 +121,35 @@ uint32_t gram_block_outputs();  // outputs per 256 x 256 block tile
 void select_topk_rerank(const SelectArgs& args, hipStream_t stream);
 uint32_t gram_fp8_k_padding();
 uint32_t gram_k_padding();
+
+// binary32 rows -> 16-bit fixed point in two int8 planes, interleaved per 64
+// columns: bytes [128 c, 128 c + 64) of a row hold a of columns 64 c .. 64 c
+// + 63 and the next 64 bytes their b; ld_dst = 128 ceil(dim / 64) bytes,
+// columns past dim are zero. n = rint(x (32512 / max|x|)) (binary32
+// division, then binary32 product), clamped to [-32512, 32512]; a = floor((n
+// + 128) / 256), b = n - 256 a; s = max|x| / 127 (binary32).
+void to_i8x2_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, uint8_t* dst, uint64_t ld_dst,
+                  I8Stats* stats, hipStream_t stream);
+// out[m][n] = binary32 of ||q||^2 + ||v||^2 - 2 s_q s_v (H + X / 256) with
+// H = <a_q, a_v> and X = <a_q, b_v> + <b_q, a_v> summed exactly in int32 on
+// v_mfma_i32_32x32x32_i8, combined in binary64. k (bytes) a multiple of 128,
+// dim <= gram_i8_max_dim(); q and v padded to whole tiles (gram_m_tile() rows).
+// tile -1: the default shape with the K split of a partial last round;
+// 0 .. gram_i8_tile_count() - 1 force a shape with every tile whole.
+void gram_i8x2(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uint64_t ldv, uint32_t n, uint32_t k,
+               const I8Stats* qs, const I8Stats* vs, float* out, uint64_t ldo, int tile, float* partials,
+               uint64_t partial_floats, hipStream_t stream);
+// Scratch floats the GRAM_INT8 K split needs for batches up to max_m.
+uint64_t gram_i8_split_floats(uint32_t max_m, uint32_t n, uint32_t k_steps, uint32_t cus);
+uint32_t gram_i8_k_padding();  // bytes per K step (64 columns, both planes)
+uint32_t gram_i8_max_dim();    // largest dim whose int32 sums cannot overflow
+int gram_i8_tile_count();
+// Certified candidate selection for GRAM_INT8: the bound
+// 2 (||q|| err_v + err_q ||v^|| + lo_q lo_v) + 2^-23 |est| + 1e-10 (||q||^2 +
+// ||v||^2) on |est - ||q - v||^2| holds for every input; rows whose lower
+// end exceeds U (1+g)^2/(1-g)^2 (U = min upper end, g the binary32 scan's
+// relative error) cannot be the binary32 answer or in its near-tie band.
+void select_rerank_i8(const SelectArgs& args, hipStream_t stream);
 int gram_tile_count();
 uint32_t gram_m_tile();
 

@@ -109,16 +109,7 @@ void index_launch(const uhann_index& ix, Workspace& ws, const float* queries, ui
 {
     const uint32_t n = static_cast<uint32_t>(count);
     if (ix.kind == UHANN_KIND_EXACT) {
-        if (strategy == UHANN_STRATEGY_DIRECT) {
-            scan_launch(*ix.scan, queries, ld_q, count, ws.keys.data(), out_idx, out_dist, stream, padded,
-                        nullptr, &ws);
-        } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
-            mixed_launch(*ix.scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
-        } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {
-            fp8_launch(*ix.scan, ws, queries, ld_q, count, out_idx, out_dist, stream);
-        } else {
-            throw std::invalid_argument("unknown query strategy");
-        }
+        run_strategy(*ix.scan, ws, queries, ld_q, padded, count, out_idx, out_dist, stream, strategy);
         return;
     }
     int* flags = ws.invalid.data();
@@ -126,16 +117,7 @@ void index_launch(const uhann_index& ix, Workspace& ws, const float* queries, ui
     if (ix.kind == UHANN_KIND_FLAT) {
         kern::gather_columns(queries, ld_q, n, ix.cols.data(), ix.scales.data(), static_cast<uint32_t>(ix.k),
                              gathered, ix.k_ld, flags, stream);
-        if (strategy == UHANN_STRATEGY_DIRECT) {
-            scan_launch(*ix.scan, gathered, ix.k_ld, count, ws.keys.data(), out_idx, out_dist, stream, true,
-                        nullptr);
-        } else if (strategy == UHANN_STRATEGY_GRAM_BF16) {
-            mixed_launch(*ix.scan, ws, gathered, ix.k_ld, count, out_idx, out_dist, stream);
-        } else if (strategy == UHANN_STRATEGY_GRAM_FP8) {
-            fp8_launch(*ix.scan, ws, gathered, ix.k_ld, count, out_idx, out_dist, stream);
-        } else {
-            throw std::invalid_argument("unknown query strategy");
-        }
+        run_strategy(*ix.scan, ws, gathered, ix.k_ld, true, count, out_idx, out_dist, stream, strategy);
         return;
     }
     // hierarchical
@@ -369,7 +351,8 @@ int uhann_index_enable_mixed(uhann_index* ix, int strategy)
 {
     return guarded([&] {
         if (ix == nullptr) throw std::invalid_argument("index handle must not be null");
-        if (strategy != UHANN_STRATEGY_GRAM_BF16 && strategy != UHANN_STRATEGY_GRAM_FP8) {
+        if (strategy != UHANN_STRATEGY_GRAM_BF16 && strategy != UHANN_STRATEGY_GRAM_FP8 &&
+            strategy != UHANN_STRATEGY_GRAM_INT8) {
             throw std::invalid_argument("unknown mixed strategy");
         }
         if (ix->kind == UHANN_KIND_HIERARCHICAL) {

@@ -25,6 +25,7 @@ STRATEGY_DIRECT = 0
 STRATEGY_GRAM_BF16 = 1
 STRATEGY_PROJECTION_BF16 = 2
 STRATEGY_GRAM_FP8 = 3
+STRATEGY_GRAM_INT8 = 4
 KIND_EXACT, KIND_FLAT, KIND_HIERARCHICAL = 0, 1, 2
 
 SAMPLE_DTYPE = np.dtype([("source_column", "<u8"), ("multiplicity", "<u8"), ("inverse_probability", "<f8")])
