@@ -8,10 +8,13 @@ one batch through the engine's C ABI with queries and references already
 resident in HBM. The timed region is bracketed by torch.cuda.synchronize()
 and the per-step latency comes from HIP events on the launch stream.
 
-`value` is the mixed-precision strategy (BF16 Gram on MFMA, exact binary32
-re-rank of a bounded candidate set, binary64 near ties); the plain binary32
-direct path is timed beside it on the same batch, and both are checked
-against the binary64 reference CPU path on a query sample.
+`value` is the certified mixed-precision strategy GRAM_INT8 (rows as 16-bit
+fixed point in two int8 planes, exact int32 sums on MFMA, a deterministic
+candidate bound, exact binary32 re-rank, binary64 near ties). The plain
+binary32 direct path, the BF16 Gram (statistical candidate bound) and the
+FP8 shortlist are timed beside it on the same batch; every strategy's answers
+are scored against binary64 distances of all queries to all references,
+computed on the GPU in the reference CPU path's order.
 
 Prints ONE JSON line (metric, value = queries/s, roofline of the dominant
 kernel, CPU baseline timed on this host with the reference's CPU path).
@@ -31,13 +34,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "queries/s at r=7028 d=60660, mixed vs FP32 at equal argmin agreement; p99 ms"
-PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "hbm": 8000.0}
+PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "int8": 5000.0, "hbm": 8000.0}
 
-# HBM bytes per launch of the Gram kernel, from two rocprofv3 --pmc runs
+# HBM bytes per launch of the int8 Gram kernel, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy kernel's readings
-# (profiles/exact_l2_r7028_d60660_b4096_pmc_traffic.md). Keyed by
+# (profiles/exact_l2_r7028_d60660_b4096_int8_pmc_traffic.md). Keyed by
 # (references, dim, batch); other shapes report null.
-GRAM_TRAFFIC_BYTES = {(7028, 60660, 4096): 5.220e9 + 0.291e9}
+GRAM_TRAFFIC_BYTES = {}
 
 
 def log(msg):
@@ -45,44 +48,89 @@ def log(msg):
 
 
 def cpu_threads() -> int:
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the job's CPU
+    share), else every CPU this process may run on."""
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
         return int(env)
-    return max(1, min(16, os.cpu_count() or 1))
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
+def host_info() -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "threads_used": cpu_threads()}
+
+
+def ref_cpu_exact_l2(refs_np, queries_np, strategy):
+    """The reference's own CPU ExactL2Index (oracle/_ref) over queries_np, in
+    its benchmark's timed scope (query_batch around the query set, build
+    untimed); strategy 0 sequential, 1 parallel_queries. Returns (seconds,
+    argmins) or None without the reference build."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    q = np.ascontiguousarray(queries_np, np.float32)
+    r, d = refs_np.shape
+    b = q.shape[0]
+    idx = np.zeros(b, np.uint64)
+    sec = np.zeros(1, np.float64)
+    rc = oracle.ref().ref_exact_l2_timed(oracle._p(refs_np, oracle._f32p), r, d, oracle._p(q, oracle._f32p), b, b,
+                                         strategy, oracle._p(idx, oracle._u64p), oracle._p(sec, oracle._f64p))
+    if rc != 0:
+        raise RuntimeError("reference CPU scan failed")
+    return float(sec[0]), idx
 
 
 def cpu_baseline(refs_np, queries_np, sample):
-    """Reference CPU exact L2 (oracle/_ref, OpenMP parallel_queries) on a
-    bounded query sample; also the binary64 argmin for agreement."""
+    """Reference CPU exact L2 (oracle/_ref, OpenMP parallel_queries on every
+    thread of the job) on a bounded query sample; the restated FP64 scan
+    (oracle) when the reference build is absent."""
     import ctypes
 
     import oracle
 
     threads = cpu_threads()
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     q = np.ascontiguousarray(queries_np[:sample])
     r, d = refs_np.shape
-    idx = np.zeros(sample, np.uint64)
+    res = ref_cpu_exact_l2(refs_np, q, 1)
     kind = "reference"
-    t0 = time.perf_counter()
-    if oracle.ref_available():
-        rc = oracle.ref().ref_exact_l2(oracle._p(refs_np, oracle._f32p), r, d, oracle._p(q, oracle._f32p),
-                                       sample, 1, oracle._p(idx, oracle._u64p))
-        if rc != 0:
-            raise RuntimeError("reference CPU scan failed")
-    else:
+    if res is None:
         kind = "port"
+        idx = np.zeros(sample, np.uint64)
+        t0 = time.perf_counter()
         oracle.lib().orc_l2_fp64_scan(oracle._p(refs_np, oracle._f32p), r, d, oracle._p(q, oracle._f32p),
                                       sample, threads, oracle._p(idx, oracle._u64p),
                                       ctypes.POINTER(ctypes.c_double)())
-    elapsed = time.perf_counter() - t0
+        elapsed = time.perf_counter() - t0
+    else:
+        elapsed, idx = res
+    info = host_info()
     return {
         "value": sample / elapsed,
         "unit": "queries/s",
         "cores": threads,
         "kind": kind,
-        "sample": f"{sample} queries of the same batch, exact L2 binary64 over r={r} d={d}, "
-                  f"OpenMP parallel_queries, {elapsed:.2f} s wall",
+        "sample": f"{sample} queries of the same batch, exact L2 binary64 over r={r} d={d}, OpenMP "
+                  f"parallel_queries on {threads} threads, query_batch timed ({elapsed:.2f} s), build untimed",
+        "host": info,
     }, idx
 
 
@@ -90,22 +138,53 @@ def cpu_baseline_sequential(refs_np, queries_np, sample, expect_idx):
     """The reference's published methodology: its `sequential` strategy on
     one thread (oracle/_ref), on a few queries of the batch; None when the
     reference build is absent."""
-    import oracle
-
-    if not oracle.ref_available():
+    res = ref_cpu_exact_l2(refs_np, queries_np[:sample], 0)
+    if res is None:
         return None
-    q = np.ascontiguousarray(queries_np[:sample])
-    r, d = refs_np.shape
-    idx = np.zeros(sample, np.uint64)
-    t0 = time.perf_counter()
-    rc = oracle.ref().ref_exact_l2(oracle._p(refs_np, oracle._f32p), r, d, oracle._p(q, oracle._f32p), sample, 0,
-                                   oracle._p(idx, oracle._u64p))
-    elapsed = time.perf_counter() - t0
-    if rc != 0:
-        return {"error": "reference sequential CPU scan failed"}
+    elapsed, idx = res
     return {"value": sample / elapsed, "unit": "queries/s", "cores": 1, "kind": "reference",
-            "sample": f"{sample} queries, exact L2 binary64, sequential strategy, {elapsed:.2f} s wall",
+            "sample": f"{sample} queries, exact L2 binary64, sequential strategy, {elapsed:.2f} s",
             "same_argmin_as_parallel": bool(np.array_equal(idx, expect_idx[:sample]))}
+
+
+def fp64_truth(torch, scan, queries, count):
+    """Binary64 distances of every query to every reference on the GPU, in
+    the reference CPU path's order and bit-identical to it (L2: square roots,
+    ExactDistanceTable, benchmark_metrics.cpp:112-192): (table, binary64
+    argmin per query, lowest row on ties)."""
+    r = scan.rows if hasattr(scan, "rows") else None
+    table = torch.empty((count, r), dtype=torch.float64, device="cuda")
+    for s in range(0, count, 1024):
+        e = min(count, s + 1024)
+        scan.distance_table(queries[s:e], e - s, table[s:e])
+    torch.cuda.synchronize()
+    return table, torch.argmin(table, dim=1)
+
+
+def score(torch, table, best, idx, dist=None, l1=False):
+    """Agreement with the binary64 argmin, relative distance excess
+    100 (returned / optimum - 1) with binary64 distances (the reference's
+    metrics, benchmark_metrics.cpp:302-391, README.md:351-353) and, with the
+    returned binary32 distances, their largest relative error against the
+    binary64 distance of the same row."""
+    gi = torch.as_tensor(np.asarray(idx).astype(np.int64), device="cuda")
+    rows = torch.arange(gi.numel(), device="cuda")
+    got = table[rows, gi]
+    opt = table[rows, best]
+    excess = torch.where(opt > 0, got / opt - 1.0, torch.zeros_like(got)) * 100.0
+    out = {
+        "queries": int(gi.numel()),
+        "agreement": round(float((gi == best).double().mean()), 6),
+        "mean_excess_pct": float(excess.mean()),
+        "max_excess_pct": float(excess.max()),
+        "over_10pct": int((excess > 10.0).sum()),
+    }
+    if dist is not None:
+        dd = torch.as_tensor(np.asarray(dist, np.float64), device="cuda")
+        exact = got if l1 else got * got
+        rel = torch.where(exact > 0, (dd - exact).abs() / exact, (dd - exact).abs())
+        out["max_rel_distance_error"] = float(rel.max())
+    return out
 
 
 def timed(fn, steps, warmup, torch):
@@ -254,6 +333,21 @@ def centroids_r33(uh, torch, args):
     out = {"references": 33, "queries": B, "dim": args.dim,
            "exact_fp32": {"queries_per_s": round(B / float(np.median(ms) * 1e-3), 1)}}
     probs = uh.importance_probabilities_device(uh.METRIC_L2, refs_np)
+    q_np = q.cpu().numpy()
+    import oracle
+
+    have_ref = oracle.ref_available()
+    if have_ref:
+        # The reference's CPU path (oracle/_ref, its own FlatL2AnnIndex and
+        # ExactL2Index) in its benchmark's timed scope: query_batch over the
+        # query set in batches of 128, build untimed.
+        for name, strat, threads in (("sequential", 0, 1), ("parallel_queries", 1, cpu_threads())):
+            os.environ["OMP_NUM_THREADS"] = str(threads)
+            sec, ex_idx = ref_cpu_exact_l2(refs_np, q_np, strat)
+            out["exact_fp32"][f"cpu_reference_{name}"] = {
+                "queries_per_s": round(B / sec, 1), "threads": threads,
+                "same_argmin_as_gpu": float(np.mean(ex_idx == exact_idx))}
+        os.environ["OMP_NUM_THREADS"] = str(cpu_threads())
     per_seed = {}
     for seed in (7, 20, 42, 1312, 2026):
         index = uh.HipFlatL2AnnIndex(refs_np, probs, 8, uh.Rng(seed))
@@ -264,20 +358,39 @@ def centroids_r33(uh, torch, args):
                "queries_per_s": round(B / float(np.median(ms) * 1e-3), 1),
                "us_per_query": round(float(np.median(ms)) * 1e3 / B, 4)}
         row.update(quality(torch, cents, q, got, exact_idx, False))
+        if have_ref:
+            pr = np.ascontiguousarray(probs, np.float64)
+            for name, strat, threads in (("sequential", 0, 1), ("parallel_queries", 1, cpu_threads())):
+                os.environ["OMP_NUM_THREADS"] = str(threads)
+                cidx = np.zeros(B, np.uint64)
+                sec = np.zeros(1, np.float64)
+                rc = oracle.ref().ref_flat_l2_timed(
+                    oracle._p(refs_np, oracle._f32p), refs_np.shape[0], refs_np.shape[1], oracle._p(pr, oracle._f64p),
+                    8, seed, oracle._p(np.ascontiguousarray(q_np), oracle._f32p), B, 128, strat,
+                    oracle._p(cidx, oracle._u64p), oracle._p(sec, oracle._f64p))
+                if rc == 0:
+                    row[f"cpu_reference_{name}"] = {"queries_per_s": round(B / float(sec[0]), 1), "threads": threads,
+                                                    "same_argmin_as_gpu": float(np.mean(cidx == got))}
+            os.environ["OMP_NUM_THREADS"] = str(cpu_threads())
         per_seed[str(seed)] = row
     out["flat_T8"] = per_seed
     out["flat_T8_mean"] = {k: round(float(np.mean([v[k] for v in per_seed.values()])), 5)
                            for k in ("queries_per_s", "agreement", "mean_excess_pct", "over_10pct")}
+    if have_ref:
+        for name in ("sequential", "parallel_queries"):
+            vals = [v[f"cpu_reference_{name}"]["queries_per_s"] for v in per_seed.values()
+                    if f"cpu_reference_{name}" in v]
+            if vals:
+                out["flat_T8_mean"][f"cpu_reference_{name}_queries_per_s"] = round(float(np.mean(vals)), 1)
     return out
 
 
-def mixed_large(uh, torch, args, r=65536, B=8192, sample=16):
+def mixed_large(uh, torch, args, r=65536, B=8192):
     """Mixed-precision exact L2 at r = 65536 synthetic points (FP32 copy
-    15.9 GB, BF16 copy 8.0 GB), d = 60660, batch 8192: BF16 Gram + certified
-    candidate re-rank vs the FP32 direct scan, argmin agreement and distance
-    error against the binary64 oracle on a query sample (16 threads)."""
-    import oracle
-
+    15.9 GB; int8 planes 8.0 GB, BF16 8.0 GB, FP8 4.0 GB), d = 60660, batch
+    8192: the certified INT8 Gram, the statistical BF16 Gram and the FP8
+    shortlist against the FP32 direct scan, every strategy scored on all 8192
+    queries against binary64 distances computed on the GPU."""
     from ultrahigh_ann_amd import synthetic
 
     d = args.dim
@@ -287,49 +400,39 @@ def mixed_large(uh, torch, args, r=65536, B=8192, sample=16):
     del refs
     t0 = time.perf_counter()
     index = uh.HipExactL2Index(refs_np)
-    index.enable_mixed()
+    for strat in (uh.STRATEGY_GRAM_INT8, uh.STRATEGY_GRAM_BF16, uh.STRATEGY_GRAM_FP8):
+        index.enable_mixed(strat)
     ws = index.make_query_workspace(B)
-    log(f"config r={r} B={B}: index + BF16 copy built in {time.perf_counter() - t0:.1f} s")
+    log(f"config r={r} B={B}: index + int8 / bf16 / fp8 copies built in {time.perf_counter() - t0:.1f} s")
     oi = torch.zeros(B, dtype=torch.int64, device="cuda")
     od = torch.zeros(B, dtype=torch.float32, device="cuda")
     wall32, ms32 = timed(lambda: index.query_device_async(queries, B, ws, oi, od), 1, 0, torch)
-    idx32 = oi.cpu().numpy().astype(np.uint64)
+    idx32, dist32 = oi.cpu().numpy().astype(np.uint64), od.cpu().numpy()
     steps = 3
-    wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od, strategy=uh.STRATEGY_GRAM_BF16),
-                     steps, 1, torch)
-    idx16 = oi.cpu().numpy().astype(np.uint64)
-    dist16 = od.cpu().numpy()
-    stats = ws.mixed_stats()
-    index.enable_mixed(uh.STRATEGY_GRAM_FP8)
-    wall8, ms8 = timed(lambda: index.query_device_async(queries, B, ws, oi, od, strategy=uh.STRATEGY_GRAM_FP8),
-                       steps, 1, torch)
-    idx8 = oi.cpu().numpy().astype(np.uint64)
-    dist8 = od.cpu().numpy()
-    cert8 = ws.shortlist_stats()["certified"]
+    res = {}
+    for name, strat in (("int8_certified", uh.STRATEGY_GRAM_INT8), ("bf16_statistical", uh.STRATEGY_GRAM_BF16),
+                        ("fp8_shortlist", uh.STRATEGY_GRAM_FP8)):
+        wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od, strategy=strat), steps, 1, torch)
+        st = ws.mixed_stats()
+        cert = ws.shortlist_stats()["certified"]
+        res[name] = {"idx": oi.cpu().numpy().astype(np.uint64), "dist": od.cpu().numpy(),
+                     "row": {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
+                             "speedup_vs_fp32": round((B * steps / wall) / (B / wall32), 2),
+                             "avg_candidates": round(st["candidates"] / B, 3), "escalated": st["escalated"],
+                             "certified_fraction": None if name == "bf16_statistical" else round(cert / B, 4)}}
     t0 = time.perf_counter()
-    e_idx, e_dist = oracle.l2_fp64_scan(refs_np, queries[:sample].cpu().numpy(), cpu_threads())
-    cpu_s = time.perf_counter() - t0
-    rel = np.abs(dist16[:sample].astype(np.float64) - e_dist) / np.maximum(e_dist, 1e-300)
-    rel8 = np.abs(dist8[:sample].astype(np.float64) - e_dist) / np.maximum(e_dist, 1e-300)
+    table, best = fp64_truth(torch, index, queries, B)
+    truth_s = time.perf_counter() - t0
     out = {"references": r, "dim": d, "batch": B,
-           "fp32_direct": {"queries_per_s": round(B / wall32, 1), "ms_per_batch": round(float(ms32.mean()), 2)},
-           "bf16_mixed": {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
-                          "gram_tflops": round(2.0 * B * r * d / (float(ms.mean()) * 1e-3) / 1e12, 1),
-                          "avg_candidates": round(stats["candidates"] / B, 3),
-                          "escalated": stats["escalated"]},
-           "fp8_shortlist": {"queries_per_s": round(B * steps / wall8, 1),
-                             "ms_per_batch": round(float(ms8.mean()), 3),
-                             "shortlist": 16, "same_argmin_as_fp32": float(np.mean(idx8 == idx32)),
-                             "certified_fraction": round(cert8 / B, 4)},
-           "speedup_mixed_vs_fp32": round((B * steps / wall) / (B / wall32), 2),
-           "speedup_fp8_vs_fp32": round((B * steps / wall8) / (B / wall32), 2),
-           "same_argmin_mixed_vs_fp32": float(np.mean(idx16 == idx32)),
-           "agreement_fp64": {"sample": sample, "fp32_direct": float(np.mean(idx32[:sample] == e_idx)),
-                              "bf16_mixed": float(np.mean(idx16[:sample] == e_idx)),
-                              "fp8_shortlist": float(np.mean(idx8[:sample] == e_idx)),
-                              "max_rel_distance_error": float(rel.max()),
-                              "max_rel_distance_error_fp8": float(rel8.max()), "cpu_seconds": round(cpu_s, 1)}}
-    del index, ws, queries
+           "fp32_direct": {"queries_per_s": round(B / wall32, 1), "ms_per_batch": round(float(ms32.mean()), 2),
+                           "fp64": score(torch, table, best, idx32, dist32)},
+           "fp64_table_seconds": round(truth_s, 2)}
+    for name, v in res.items():
+        row = v["row"]
+        row["same_argmin_as_fp32"] = float(np.mean(v["idx"] == idx32))
+        row["fp64"] = score(torch, table, best, v["idx"], v["dist"])
+        out[name] = row
+    del index, ws, queries, table
     torch.cuda.empty_cache()
     return out
 
@@ -353,12 +456,27 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
             res[name] = {"error": f"{type(exc).__name__}: {exc}"}
             log(f"suite {name} failed: {exc}")
 
+    truth_l1 = {}
+
+    def exact_idx_l1():
+        # binary64 L1 argmin of every query (GPU table in the reference's
+        # transform_reduce order)
+        if "idx" not in truth_l1:
+            scan = uh.HipExactL1Index(refs_np)
+            table, best = fp64_truth(torch, scan, queries, B)
+            truth_l1["idx"] = best.cpu().numpy().astype(np.uint64)
+            del table, scan
+            torch.cuda.empty_cache()
+        return truth_l1["idx"]
+
     def exact_l1():
         index = uh.HipExactIndex(refs_np, uh.METRIC_L1)
         ws = index.make_query_workspace(B)
         wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od), steps, 1, torch)
+        got = oi.cpu().numpy().astype(np.uint64)
         return {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
-                "tflops_2flop": round(2.0 * r * d * B / (ms.mean() * 1e-3) / 1e12, 2)}
+                "tflops_2flop": round(2.0 * r * d * B / (ms.mean() * 1e-3) / 1e12, 2),
+                "agreement_fp64": float(np.mean(got == exact_idx_l1()))}
 
     def streaming():
         index = uh.HipExactIndex(refs_np, uh.METRIC_L2)
@@ -405,14 +523,16 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
             row = {"unique_coordinates": use["unique_query_coordinates"],
                    "queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
                    "tile": [tuned["rpt"], tuned["tq"], tuned["tr"], tuned["group_q"]]}
-            if exact_idx_l2 is not None and not l1:
-                row.update(quality(torch, refs, queries, got, exact_idx_l2, False))
+            row.update(quality(torch, refs, queries, got, exact_idx_l1() if l1 else exact_idx_l2, l1))
             if not l1:
-                index.enable_mixed()
-                wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od,
-                                                                  strategy=uh.STRATEGY_GRAM_BF16), steps, 1, torch)
-                row["mixed_queries_per_s"] = round(B * steps / wall, 1)
-                row["mixed_same_as_direct"] = round(float(np.mean(oi.cpu().numpy().astype(np.uint64) == got)), 5)
+                for label, strat in (("int8_certified", uh.STRATEGY_GRAM_INT8),
+                                     ("bf16_statistical", uh.STRATEGY_GRAM_BF16)):
+                    index.enable_mixed(strat)
+                    wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od, strategy=strat),
+                                     steps, 1, torch)
+                    row[label] = {"queries_per_s": round(B * steps / wall, 1),
+                                  "same_as_direct": round(float(np.mean(oi.cpu().numpy().astype(np.uint64) == got)),
+                                                          5)}
             rows[f"T{T}"] = row
         return rows
 
@@ -431,8 +551,7 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
             got = oi.cpu().numpy().astype(np.uint64)
             row = {"unique_coordinates": index.space_usage()["unique_query_coordinates"],
                    "queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3)}
-            if exact_idx_l2 is not None and not l1:
-                row.update(quality(torch, refs, queries, got, exact_idx_l2, False))
+            row.update(quality(torch, refs, queries, got, exact_idx_l1() if l1 else exact_idx_l2, l1))
             if not l1:
                 # projection GEMM in bf16 on MFMA (binary32 accumulation)
                 wall, ms = timed(lambda: index.query_device_async(queries, B, ws, oi, od,
@@ -441,8 +560,7 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
                 g16 = oi.cpu().numpy().astype(np.uint64)
                 bf = {"queries_per_s": round(B * steps / wall, 1), "ms_per_batch": round(float(ms.mean()), 3),
                       "same_as_fp32": round(float(np.mean(g16 == got)), 5)}
-                if exact_idx_l2 is not None:
-                    bf.update(quality(torch, refs, queries, g16, exact_idx_l2, False))
+                bf.update(quality(torch, refs, queries, g16, exact_idx_l2, False))
                 row["bf16_projection"] = bf
             rows[f"T32_p{p}"] = row
         return rows
@@ -504,9 +622,10 @@ def main():
     log(f"generating synthetic data r={r} d={d} B={B} seed={args.seed}")
     refs, queries = synthetic.log_expression(r, B, d, seed=args.seed, device="cuda")
     refs_np = refs.cpu().numpy()
-    log("building index (upload references, BF16 copy)")
+    log("building index (upload references, int8 / bf16 / fp8 copies)")
     index = uh.HipExactL2Index(refs_np)
-    index.enable_mixed()
+    for strat in (uh.STRATEGY_GRAM_INT8, uh.STRATEGY_GRAM_BF16, uh.STRATEGY_GRAM_FP8):
+        index.enable_mixed(strat)
     ws = index.make_query_workspace(B)
     stream = torch.cuda.current_stream()
     out_idx = torch.zeros(B, dtype=torch.int64, device="cuda")
@@ -527,75 +646,90 @@ def main():
         wall = time.perf_counter() - t0
         ms = np.array([events[i].elapsed_time(events[i + 1]) for i in range(steps)])
         log(f"{label}: {steps} steps {wall:.3f} s wall, per-step ms {np.round(ms, 3).tolist()}")
-        return wall, ms, out_idx.cpu().numpy().astype(np.uint64)
+        return wall, ms, out_idx.cpu().numpy().astype(np.uint64), out_dist.cpu().numpy().copy()
+
+    def stages(strategy):
+        # Per-stage HIP-event times (same stream), outside the timed region.
+        ws.set_timing(True)
+        st = []
+        for _ in range(3):
+            index.query_device_async(queries, B, ws, out_idx, out_dist, strategy=strategy, stream=stream)
+            st.append(ws.stage_times())
+        ws.set_timing(False)
+        return np.array(st).mean(0)
 
     fp32_steps = max(1, min(args.steps, 3))
-    fp32_wall, fp32_ms, fp32_idx = run(uh.STRATEGY_DIRECT, fp32_steps, 1, "fp32 direct")
-    K = args.steps
-    wall, step_ms, mixed_idx = run(uh.STRATEGY_GRAM_BF16, K, args.warmup, "bf16 mixed")
-    stats = ws.mixed_stats()
-    qps = B * K / wall
+    fp32_wall, fp32_ms, fp32_idx, fp32_dist = run(uh.STRATEGY_DIRECT, fp32_steps, 1, "fp32 direct")
     fp32_qps = B * fp32_steps / fp32_wall
-
-    # Per-stage HIP-event times of the mixed strategy (same stream), outside
-    # the timed region.
-    ws.set_timing(True)
-    stage = []
-    for _ in range(3):
-        index.query_device_async(queries, B, ws, out_idx, out_dist, strategy=uh.STRATEGY_GRAM_BF16,
-                                 stream=stream)
-        stage.append(ws.stage_times())
-    ws.set_timing(False)
-    stage = np.array(stage).mean(0)
+    K = args.steps
+    wall, step_ms, i8_idx, i8_dist = run(uh.STRATEGY_GRAM_INT8, K, args.warmup, "int8 certified")
+    stats = ws.mixed_stats()
+    cert = ws.shortlist_stats()["certified"]
+    qps = B * K / wall
+    stage = stages(uh.STRATEGY_GRAM_INT8)
     gemm_ms = float(stage[1])
 
-    # FP8 shortlist strategy on the same batch (not certified; reported beside).
-    index.enable_mixed(uh.STRATEGY_GRAM_FP8)
-    wall8, ms8, idx8 = run(uh.STRATEGY_GRAM_FP8, K, args.warmup, "fp8 shortlist")
+    wall16, ms16, idx16, dist16 = run(uh.STRATEGY_GRAM_BF16, K, args.warmup, "bf16 statistical")
+    stats16 = ws.mixed_stats()
+    stage16 = stages(uh.STRATEGY_GRAM_BF16)
+    wall8, ms8, idx8, dist8 = run(uh.STRATEGY_GRAM_FP8, K, args.warmup, "fp8 shortlist")
     cert8 = ws.shortlist_stats()["certified"]
-    ws.set_timing(True)
-    stage8 = []
-    for _ in range(3):
-        index.query_device_async(queries, B, ws, out_idx, out_dist, strategy=uh.STRATEGY_GRAM_FP8, stream=stream)
-        stage8.append(ws.stage_times())
-    ws.set_timing(False)
-    stage8 = np.array(stage8).mean(0)
-    fp8_line = {"value": round(B * K / wall8, 2), "unit": "queries/s", "ms_per_step": round(wall8 * 1e3 / K, 3),
-                "same_argmin_as_fp32": float(np.mean(idx8 == fp32_idx)), "certified_fraction": round(cert8 / B, 4),
-                "stages_ms": {"fp8_convert": round(float(stage8[0]), 3), "gram_gemm": round(float(stage8[1]), 3),
-                              "top16_rerank": round(float(stage8[2]), 3)},
-                "gram_tflops": round(2.0 * B * r * d / (float(stage8[1]) * 1e-3) / 1e12, 1)}
-    flops = 2.0 * B * r * d
-    achieved = flops / (gemm_ms * 1e-3) / 1e12
+    stage8 = stages(uh.STRATEGY_GRAM_FP8)
+
+    # Binary64 truth for every query of the batch (GPU, reference CPU order).
+    t0 = time.perf_counter()
+    table, best = fp64_truth(torch, index, queries, B)
+    truth_s = time.perf_counter() - t0
+    quality_fp64 = {"fp32_direct": score(torch, table, best, fp32_idx, fp32_dist),
+                    "int8_certified": score(torch, table, best, i8_idx, i8_dist),
+                    "bf16_statistical": score(torch, table, best, idx16, dist16),
+                    "fp8_shortlist": score(torch, table, best, idx8, dist8),
+                    "table_seconds": round(truth_s, 3)}
+    exact_idx_l2 = best.cpu().numpy().astype(np.uint64)
+    del table
+    log(f"binary64 scores: {quality_fp64}")
+
+    # The reference's timed scope: host queries in, host results out
+    # (H2D of the queries, the query, D2H), README.md:139-147.
+    q_host = queries.cpu().numpy()
+    index.query_batch(q_host, B, ws, uh.STRATEGY_GRAM_INT8)
+    t0 = time.perf_counter()
+    h_idx, _ = index.query_batch(q_host, B, ws, uh.STRATEGY_GRAM_INT8)
+    host_scope_s = time.perf_counter() - t0
+    host_scope = {"queries_per_s": round(B / host_scope_s, 1), "ms_per_batch": round(host_scope_s * 1e3, 3),
+                  "same_as_device_resident": bool(np.array_equal(h_idx, i8_idx)),
+                  "scope": "pageable host queries -> H2D -> int8 certified query -> D2H results"}
+
+    ops = 3 * 2.0 * B * r * d  # three int8 products (<a,a>, <a,b>, <b,a>), 2 ops per multiply-add
+    achieved = ops / (gemm_ms * 1e-3) / 1e12
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 2),
-        "peak": PEAK["bf16"],
+        "peak": PEAK["int8"],
         "unit": "TFLOP/s",
-        "frac": round(achieved / PEAK["bf16"], 4),
+        "frac": round(achieved / PEAK["int8"], 4),
         "traffic": GRAM_TRAFFIC_BYTES.get((r, d, B)),
+        "kernel": "gram_i8x2_kernel (+ tail fix-up), dense int8 MFMA peak 5.0 POP/s; ops are int8 operations",
     }
 
     cpu = None
     cpu_seq = None
-    agreement = None
     if not args.no_cpu:
         threads = cpu_threads()
         sample = args.cpu_sample or max(8, 2 * threads)
         log(f"CPU baseline: {sample} queries on {threads} threads")
-        cpu, fp64_idx = cpu_baseline(refs_np, queries.cpu().numpy(), sample)
-        cpu_seq = cpu_baseline_sequential(refs_np, queries.cpu().numpy(), 2, fp64_idx)
-        agreement = {
-            "sample": sample,
-            "fp32_direct": float(np.mean(fp32_idx[:sample] == fp64_idx)),
-            "bf16_mixed": float(np.mean(mixed_idx[:sample] == fp64_idx)),
-            "fp8_shortlist": float(np.mean(idx8[:sample] == fp64_idx)),
-        }
-        log(f"CPU baseline {cpu['value']:.2f} q/s; agreement with binary64 {agreement}")
+        cpu, fp64_idx = cpu_baseline(refs_np, q_host, sample)
+        cpu["same_argmin_as_gpu_fp64"] = bool(np.array_equal(fp64_idx, exact_idx_l2[:sample]))
+        cpu_seq = cpu_baseline_sequential(refs_np, q_host, 2, fp64_idx)
+        log(f"CPU baseline {cpu['value']:.2f} q/s")
 
     workloads = None
     if not args.no_suite:
-        workloads = suite(uh, torch, refs, queries, refs_np, mixed_idx, args)
+        workloads = suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args)
+
+    def line_of(w, k, st):
+        return {"value": round(B * K / w, 2), "unit": "queries/s", "ms_per_step": round(w * 1e3 / K, 3),
+                "same_argmin_as_fp32": float(np.mean(k == fp32_idx)), "stages_ms": st}
 
     line = {
         "metric": METRIC,
@@ -608,30 +742,44 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": "int8",
         "data": "synthetic",
         "config": {
-            "workload": f"exact-l2 r={r} d={d} batch={B}: BF16 Gram on MFMA + exact re-rank",
+            "workload": f"exact-l2 r={r} d={d} batch={B}: certified int8 fixed-point Gram on MFMA + exact re-rank",
             "references": r,
             "dim": d,
             "batch": B,
-            "strategy": "gram-bf16 (fp32 re-rank, fp64 near ties)",
+            "strategy": "gram-int8 (16-bit fixed point, exact int32 sums, deterministic bound, fp32 re-rank, "
+                        "fp64 near ties)",
         },
         "p50_ms": round(float(np.percentile(step_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(step_ms, 99)), 3),
+        "same_argmin_mixed_vs_fp32": float(np.mean(i8_idx == fp32_idx)),
+        "certified_fraction": round(cert / B, 4),
         "fp32_direct": {
             "value": round(fp32_qps, 2),
             "unit": "queries/s",
             "ms_per_step": round(fp32_wall * 1e3 / fp32_steps, 3),
             "speedup_mixed_vs_fp32": round(qps / fp32_qps, 2),
         },
-        "mixed_stages_ms": {"bf16_convert": round(float(stage[0]), 3), "gram_gemm": round(gemm_ms, 3),
-                            "select_rerank": round(float(stage[2]), 3),
-                            "escalation": round(float(stage[3]), 3)},
-        "fp8_shortlist": fp8_line,
+        "mixed_stages_ms": {"int8_convert": round(float(stage[0]), 3), "gram_gemm": round(gemm_ms, 3),
+                            "select_rerank": round(float(stage[2]), 3), "escalation": round(float(stage[3]), 3)},
         "mixed_counters": {"avg_candidates": round(stats["candidates"] / B, 3),
                            "fp64_reranked": stats["fp64_reranked"], "escalated": stats["escalated"]},
-        "agreement_fp64": agreement,
+        "bf16_statistical": dict(line_of(wall16, idx16, {"bf16_convert": round(float(stage16[0]), 3),
+                                                          "gram_gemm": round(float(stage16[1]), 3),
+                                                          "select_rerank": round(float(stage16[2]), 3)}),
+                                 certified_fraction=None,
+                                 note="candidate bound models bf16 rounding errors as independent; not a certificate",
+                                 avg_candidates=round(stats16["candidates"] / B, 3),
+                                 gram_tflops=round(2.0 * B * r * d / (float(stage16[1]) * 1e-3) / 1e12, 1)),
+        "fp8_shortlist": dict(line_of(wall8, idx8, {"fp8_convert": round(float(stage8[0]), 3),
+                                                     "gram_gemm": round(float(stage8[1]), 3),
+                                                     "top16_rerank": round(float(stage8[2]), 3)}),
+                              certified_fraction=round(cert8 / B, 4),
+                              gram_tflops=round(2.0 * B * r * d / (float(stage8[1]) * 1e-3) / 1e12, 1)),
+        "quality_fp64_all_queries": quality_fp64,
+        "reference_timing_scope": host_scope,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_sequential": cpu_seq,

@@ -96,6 +96,10 @@ def ref():
         sig = {
             "ref_exact_l2": ([_f32p, c_uint64, c_uint64, _f32p, c_uint64, c_int, _u64p], c_int),
             "ref_exact_l1": ([_f32p, c_uint64, c_uint64, _f32p, c_uint64, _u64p], c_int),
+            "ref_exact_l2_timed": ([_f32p, c_uint64, c_uint64, _f32p, c_uint64, c_uint64, c_int, _u64p, _f64p],
+                                   c_int),
+            "ref_flat_l2_timed": ([_f32p, c_uint64, c_uint64, _f64p, c_uint64, c_uint64, _f32p, c_uint64, c_uint64,
+                                   c_int, _u64p, _f64p], c_int),
             "ref_build_coordinate_sample": ([_f64p, c_uint64, c_uint64, c_uint64, _u64p, _u64p, _f64p],
                                             c_int64),
             "ref_flat": ([c_int, _f32p, c_uint64, c_uint64, _f64p, c_uint64, c_uint64, _f32p, c_uint64,

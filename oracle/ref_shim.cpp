@@ -34,6 +34,7 @@
 #include <cstring>
 #include <limits>
 #include <exception>
+#include <chrono>
 #include <random>
 #include <span>
 #include <vector>
@@ -115,6 +116,38 @@ void put_metrics(double* o, const bm::ApproximationMetrics& m)
     put_buckets(o, m.margin_buckets);
 }
 
+uh::CpuDenseL2QueryStrategy cpu_strategy(int code)
+{
+    switch (code) {
+    case 0: return uh::CpuDenseL2QueryStrategy::sequential;
+    case 1: return uh::CpuDenseL2QueryStrategy::parallel_queries;
+    case 2: return uh::CpuDenseL2QueryStrategy::parallel_representatives;
+    default: return uh::CpuDenseL2QueryStrategy::automatic;
+    }
+}
+
+// The reference benchmark's timed scope for a CPU L2 index
+// (nearest_neighbor_benchmark.cpp:386-413 around run_cpu_l2_batched,
+// :454-466): query_batch over batches of `batch` queries with `strategy`
+// (0 sequential, 1 parallel_queries, 2 parallel_representatives, 3
+// automatic), steady_clock around the whole query set. Index build is not
+// timed. *seconds gets the wall time.
+template <class Index>
+void timed_batches(const Index& index, const float* queries, uint64_t d, uint64_t b, uint64_t batch, int strategy,
+                   uint64_t* out, double* seconds)
+{
+    std::vector<std::size_t> res(b);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t begin = 0; begin < b; begin += batch) {
+        const uint64_t count = std::min(batch, b - begin);
+        index.query_batch(std::span<const float>(queries + begin * d, count * d), count,
+                          std::span<std::size_t>(res).subspan(begin, count), cpu_strategy(strategy));
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    *seconds = std::chrono::duration<double>(t1 - t0).count();
+    for (uint64_t i = 0; i < b; ++i) out[i] = res[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -160,6 +193,30 @@ int ref_exact_l2(const float* refs, uint64_t r, uint64_t d, const float* queries
                           parallel ? uh::CpuDenseL2QueryStrategy::parallel_queries
                                    : uh::CpuDenseL2QueryStrategy::sequential);
         for (uint64_t i = 0; i < b; ++i) out[i] = res[i];
+    });
+}
+
+int ref_exact_l2_timed(const float* refs, uint64_t r, uint64_t d, const float* queries, uint64_t b,
+                       uint64_t batch, int strategy, uint64_t* out, double* seconds)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        const uh::ExactL2Index index(m);
+        timed_batches(index, queries, d, b, batch == 0 ? b : batch, strategy, out, seconds);
+    });
+}
+
+// FlatL2AnnIndex(input, probabilities, t, mt19937_64(seed)) built untimed,
+// then the timed query set as above.
+int ref_flat_l2_timed(const float* refs, uint64_t r, uint64_t d, const double* probs, uint64_t t, uint64_t seed,
+                      const float* queries, uint64_t b, uint64_t batch, int strategy, uint64_t* out,
+                      double* seconds)
+{
+    return guarded([&] {
+        const auto m = make_matrix(refs, r, d);
+        std::mt19937_64 eng(seed);
+        const uh::FlatL2AnnIndex index(m, std::span<const double>(probs, d), t, eng);
+        timed_batches(index, queries, d, b, batch == 0 ? b : batch, strategy, out, seconds);
     });
 }
 
