@@ -151,6 +151,19 @@ int uhann_scan_query_device_async(const uhann_scan* scan, uhann_workspace* ws,
                                   uint64_t count, uint64_t* device_out_idx,
                                   float* device_out_dist, int strategy, void* stream);
 
+/* uhann_scan_query_device_async with the DIRECT strategy for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops): no
+ * workspace, and no device memory is allocated or freed by the call. The
+ * argmin keys live in a fixed device array of the library (65536 queries,
+ * 512 KiB per GPU, held until the process ends); count > 65536 -> LENGTH.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams or threads) race, so order them on one stream. Same
+ * results, bit for bit, as uhann_scan_query_device_async(DIRECT). No
+ * counterpart in the reference. */
+int uhann_scan_query_device_noalloc(const uhann_scan* scan, const float* device_queries, uint64_t ld_queries,
+                                    uint64_t count, uint64_t* device_out_idx, float* device_out_dist,
+                                    void* stream);
+
 /* As uhann_scan_query_device_async with an explicit register tile: rpt
  * chains per thread (1, 2 or 4), tq x tr (query, reference) pairs per block,
  * group_q query tiles per reference sweep; rpt = 0 selects the

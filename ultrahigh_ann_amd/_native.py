@@ -164,6 +164,8 @@ def _declare(lib):
         "uhann_scan_int8_estimates": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_uint64,
                                               c_int, c_void_p]),
         "uhann_int8_rows": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p]),
+        "uhann_scan_query_device_noalloc": (c_int, [c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p,
+                                                    c_void_p]),
         "uhann_server_create": (c_int, [POINTER(ServerConfig), POINTER(c_void_p)]),
         "uhann_server_destroy": (None, [c_void_p]),
         "uhann_server_register": (c_int, [c_void_p, POINTER(IndexSpec), u64p]),

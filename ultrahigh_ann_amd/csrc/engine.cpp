@@ -884,6 +884,26 @@ int uhann_scan_query_device_async(const uhann_scan* scan, uhann_workspace* ws,
     });
 }
 
+int uhann_scan_query_device_noalloc(const uhann_scan* scan, const float* device_queries, uint64_t ld_queries,
+                                    uint64_t count, uint64_t* device_out_idx, float* device_out_dist, void* stream)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        if (count > kern::noalloc_max_queries()) {
+            throw std::length_error("the allocation-free query takes at most 65536 queries per call");
+        }
+        if (count == 0) return;
+        if (scan->dim != 0 && device_queries == nullptr) {
+            throw std::invalid_argument("device query pointer must not be null");
+        }
+        if (ld_queries < scan->dim) throw std::invalid_argument("query row stride is below dim");
+        select_device(scan->device);
+        unsigned long long* keys = kern::noalloc_keys(scan->device);
+        scan_launch(*scan, device_queries, ld_queries, count, keys, device_out_idx, device_out_dist,
+                    static_cast<hipStream_t>(stream), false, nullptr, nullptr);
+    });
+}
+
 int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
                                  const float* device_queries, uint64_t ld_queries, uint64_t count,
                                  uint64_t* device_out_idx, float* device_out_dist, int rpt, int tq,

@@ -62,6 +62,11 @@ bool scan_tile_supported(const ScanTile& tile);
 // Every tiled shape compiled in (candidates of the autotuner).
 const std::vector<ScanTile>& scan_tiles();
 void fill_keys(unsigned long long* keys, uint32_t n, hipStream_t stream);
+// The allocation-free direct query's key array (scan.hip `scratch`): its
+// address on `device`, which must be current (resolved once per device), and
+// its capacity in queries.
+unsigned long long* noalloc_keys(int device);
+uint32_t noalloc_max_queries();
 void scan_direct(Metric metric, bool weight_fma, const ScanArgs& args,
                  const ScanTile& tile, hipStream_t stream);
 void finalize_keys(const unsigned long long* keys, uint32_t n, uint64_t* out_idx,

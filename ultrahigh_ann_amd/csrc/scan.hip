@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -349,6 +350,39 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     }
 #undef UHANN_PAIRS
     check_launch("pair scan kernel");
+}
+
+
+// Argmin keys of the allocation-free direct query: a file-scope device array
+// of a fixed size, one copy per GPU, held until the process ends. Host code
+// never passes `scratch` itself to a kernel (in host code the name is only a
+// stand-in for the symbol): noalloc_keys resolves the array's address on
+// each device once, with that device current, and callers use that address.
+constexpr uint32_t kNoAllocQueries = 65536;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocQueries];
+
+uint32_t noalloc_max_queries() { return kNoAllocQueries; }
+
+unsigned long long* noalloc_keys(int device)
+{
+    static std::mutex mu;
+    static unsigned long long* ptrs[kNoAllocDevices] = {};
+    if (device < 0 || device >= kNoAllocDevices) throw std::invalid_argument("device id out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (ptrs[device] == nullptr) {
+        int current = -1;
+        if (hipGetDevice(&current) != hipSuccess || current != device) {
+            throw std::runtime_error("the scratch keys must be resolved with their device current");
+        }
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            throw std::runtime_error(std::string("cannot resolve the scratch key array: ") + hipGetErrorString(e));
+        }
+        ptrs[device] = static_cast<unsigned long long*>(p);
+    }
+    return ptrs[device];
 }
 
 }  // namespace uhann::kern
