@@ -25,16 +25,16 @@ def _batches(seed, r, d, b, escalate):
     return refs, qa, qb
 
 
-@pytest.mark.parametrize("strategy", ["direct", "gram_bf16", "gram_fp8"])
-@pytest.mark.parametrize("escalate", [False, True])
+# Escalation to the direct scan exists in the certified strategies (BF16, INT8).
+@pytest.mark.parametrize("strategy,escalate", [("direct", False), ("gram_bf16", False), ("gram_bf16", True),
+                                               ("gram_fp8", False), ("gram_int8", False), ("gram_int8", True)])
 def test_query_graph_replay(uh, strategy, escalate):
     import torch
 
-    if strategy != "gram_bf16" and escalate:
-        pytest.skip("escalation exists only in the BF16 strategy")
     r, d, b = 300, 1000, 40
     refs, qa, qb = _batches(31 + int(escalate), r, d, b, escalate)
-    strat = {"direct": uh.STRATEGY_DIRECT, "gram_bf16": uh.STRATEGY_GRAM_BF16, "gram_fp8": uh.STRATEGY_GRAM_FP8}[strategy]
+    strat = {"direct": uh.STRATEGY_DIRECT, "gram_bf16": uh.STRATEGY_GRAM_BF16, "gram_fp8": uh.STRATEGY_GRAM_FP8,
+             "gram_int8": uh.STRATEGY_GRAM_INT8}[strategy]
     index = uh.HipExactL2Index(refs)
     if strat != uh.STRATEGY_DIRECT:
         index.enable_mixed(strat)

@@ -177,17 +177,67 @@ def test_uniform_parity_bit_exact(uh, oracle_mod, metric, r, d, b, T):
     np.testing.assert_array_equal(dist.view(np.uint32), e_dist.view(np.uint32))
 
 
-def test_flat_l2_mixed_matches_direct(uh, oracle_mod):
+@pytest.mark.parametrize("strategy", ["int8", "bf16"])
+def test_flat_l2_mixed_matches_direct(uh, oracle_mod, strategy):
+    """The Gram strategies of the flat L2 index meet the exact index's bar on
+    the embedded (gathered, scaled) coordinates: the direct path's row and
+    binary32 bits, or another row only through the binary64 near-tie re-rank
+    with a binary64 distance no larger; every returned distance is the K2
+    distance of its row."""
     refs, q = _random_index_case(5, 400, 6000, 30)
     probs = oracle_mod.importance_probabilities(False, refs)
     index = uh.HipFlatL2AnnIndex(refs, probs, 128, uh.Rng(7))
-    index.enable_mixed()
+    strat = uh.STRATEGY_GRAM_INT8 if strategy == "int8" else uh.STRATEGY_GRAM_BF16
+    index.enable_mixed(strat)
     ws = index.make_query_workspace(30)
     d_idx, d_dist = index.query_batch(q, 30, ws)
-    m_idx, m_dist = index.query_batch(q, 30, ws, uh.STRATEGY_GRAM_BF16)
-    same = m_idx == d_idx
-    np.testing.assert_array_equal(m_dist[same].view(np.uint32), d_dist[same].view(np.uint32))
-    assert same.mean() >= 0.9
+    m_idx, m_dist = index.query_batch(q, 30, ws, strat)
+    cols, mult, invp = (index.sample[f] for f in ("source_column", "multiplicity", "inverse_probability"))
+    scales, _ = oracle_mod.embedding_factors(mult, invp)
+    er, eq = _emb(refs, cols, scales), _emb(q, cols, scales)
+    k2 = oracle_mod.k2_distances(er, eq)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(k2)
+    np.testing.assert_array_equal(d_idx, exp_idx)
+    rows = np.arange(30)
+    np.testing.assert_array_equal(m_dist.view(np.uint32), k2[rows, m_idx.astype(np.int64)].view(np.uint32))
+    diff = m_idx != d_idx
+    table = oracle_mod.fp64_table(False, er, eq)
+    assert np.all(table[rows[diff], m_idx[diff].astype(np.int64)] <= table[rows[diff], d_idx[diff].astype(np.int64)])
+    if strategy == "int8":
+        assert ws.shortlist_stats()["certified"] + ws.mixed_stats()["escalated"] == 30
+
+
+@pytest.mark.parametrize("seed", [7, 42])
+def test_flat_l1_matches_reference_cpu_outside_tie_band(uh, ref_mod, seed):
+    """Flat L1 on the GPU (binary32 weighted K5 order) against the
+    reference's own CPU FlatL1AnnIndex (binary64, oracle/_ref) built from the
+    same probabilities and seed: same rows wherever the binary64 gap between
+    the two nearest rows exceeds the binary32 tie band; a different row
+    otherwise is within that band of the reference's."""
+    r, d, b, T = 500, 3000, 64, 32
+    g = rng(seed)
+    refs = (g.random((r, d), dtype=np.float32) * 5).astype(F32)
+    q = (refs[g.integers(0, r, b)] + g.random((b, d), dtype=np.float32) * 2).astype(F32)
+    probs = ref_mod.importance_probabilities(True, refs)
+    ref_idx = np.zeros(b, np.uint64)
+    assert ref_mod.ref().ref_flat(1, ref_mod._p(refs, ref_mod._f32p), r, d, ref_mod._p(probs, ref_mod._f64p), T,
+                                  seed, ref_mod._p(q, ref_mod._f32p), b, ref_mod._p(ref_idx, ref_mod._u64p)) == 0
+    index = uh.HipFlatL1AnnIndex(refs, probs, T, uh.Rng(seed))
+    ws = index.make_query_workspace(b)
+    idx, _ = index.query_batch(q, b, ws)
+    cols, mult, invp = (index.sample[f] for f in ("source_column", "multiplicity", "inverse_probability"))
+    w64 = mult.astype(np.float64) * invp
+    sub_r = refs[:, cols.astype(np.int64)].astype(np.float64)
+    sub_q = q[:, cols.astype(np.int64)].astype(np.float64)
+    exact = (w64[None, None, :] * np.abs(sub_q[:, None, :] - sub_r[None, :, :])).sum(2)
+    np.testing.assert_array_equal(ref_idx, exact.argmin(1))
+    kk = -(-cols.size // 256) + 11
+    gamma = kk * 2.0**-24 / (1 - kk * 2.0**-24)
+    rows = np.arange(b)
+    diff = idx != ref_idx
+    band = (2 * gamma) * exact[rows, ref_idx.astype(np.int64)]
+    assert np.all(exact[rows[diff], idx[diff].astype(np.int64)] - exact[rows[diff], ref_idx[diff].astype(np.int64)]
+                  <= band[diff])
 
 
 def _hier_expect(oracle_mod, metric_l1, refs, q, probs, T, p, seed):

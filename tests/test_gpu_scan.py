@@ -346,3 +346,50 @@ def test_distance_table_rejects_weighted(uh):
     out = torch.zeros((1, 3), dtype=torch.float64, device="cuda")
     with pytest.raises(uh.UhannInvalidArgument):
         index.distance_table(torch.ones((1, 4), device="cuda"), 1, out)
+
+
+@pytest.mark.parametrize("tq", [1, 2])
+def test_pair_scan_batches_beyond_grid_limit(uh, oracle_mod, tq):
+    """More query tiles than one grid dimension holds (65535): the pair scan
+    runs the batch in slices. Queries around the slice boundaries and a
+    spread sample agree with the oracle bit for bit."""
+    import torch
+
+    r, d, b = 300, 222, 140000
+    g = rng(77 + tq)
+    refs = (g.random((r, d), dtype=np.float32) * 4).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 4).astype(F32)
+    index = uh.HipDenseL2ScanIndex(refs)
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    index.query_device_tile(dq, b, ws, oi, od, (0, tq, 0, 0), stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    edge = 65535 * tq
+    pick = np.unique(np.concatenate([np.arange(edge - 3, edge + 3), np.arange(2 * edge - 3, min(b, 2 * edge + 3)),
+                                     np.arange(0, b, 97), [b - 1]]))
+    pick = pick[pick < b]
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q[pick]))
+    np.testing.assert_array_equal(oi.cpu().numpy()[pick].astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy()[pick].view(np.uint32), exp_dist.view(np.uint32))
+
+
+def test_autotune_large_batch_short_rows(uh, oracle_mod):
+    """Autotuning up to 65536 queries on short rows tries the pair tiles at
+    every bucket (the tq = 1 tile needs 65536 query tiles there) and keeps
+    the bits."""
+    import torch
+
+    r, d, b = 200, 222, 65536
+    refs, q = _data(5, r, d, b)
+    index = uh.HipDenseL2ScanIndex(refs)
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    table = index.autotune(dq, b, ws)
+    assert len(table) == 17
+    idx, dist = index.query_device_batch(dq, b, ws)
+    pick = np.arange(0, b, 331)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q[pick]))
+    np.testing.assert_array_equal(idx[pick], exp_idx)
+    np.testing.assert_array_equal(dist[pick].view(np.uint32), exp_dist.view(np.uint32))

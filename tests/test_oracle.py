@@ -293,3 +293,77 @@ def test_flat_and_hierarchical_queries_match_reference(ref_mod, metric_l1):
                                           ref_mod._p(q, ref_mod._f32p), 12,
                                           ref_mod._p(theirs, ref_mod._u64p)) == 0
     np.testing.assert_array_equal(ours_h, theirs)
+
+
+def _weighted_case(seed, r=50, k=700, b=12):
+    g = rng(seed)
+    refs = (g.random((r, k), dtype=np.float32) * 6).astype(F32)
+    q = (g.random((b, k), dtype=np.float32) * 6).astype(F32)
+    mult = g.integers(1, 5, k).astype(np.uint64)
+    invp = np.exp(g.uniform(np.log(1.0), np.log(100.0), k))  # weights over more than two orders of magnitude
+    w64 = mult.astype(np.float64) * invp                     # sampled_coordinate_l1_ann_index.cpp:63-83
+    w32 = w64.astype(F32)                                    # the CUDA embedding's float weights
+    return refs, q, mult, invp, w64, w32
+
+
+def test_weighted_k5_within_gamma_of_binary64(oracle_mod):
+    """The weighted K5 emulation (both contraction modes) stays within the
+    binary32 error bound gamma_k (k = ceil(dim / 256) + 11: the chain, the
+    8-level tree, the subtract, the product and the weight's rounding to
+    binary32) of the reference's binary64 weighted L1; dropping the weights
+    or applying them to the wrong columns leaves that bound."""
+    refs, q, mult, invp, w64, w32 = _weighted_case(3)
+    k = refs.shape[1]
+    exact = (w64[None, None, :] * np.abs(q.astype(np.float64)[:, None, :] - refs.astype(np.float64)[None, :, :])).sum(2)
+    # numpy's binary64 sum agrees with the oracle's (pinned to the reference's
+    # SampledCoordinateL1AnnIndex) at the argmin
+    cols = np.arange(k, dtype=np.uint64)
+    e_idx, e_dist = oracle_mod.weighted_l1_fp64_scan(refs, q, cols, mult, invp)
+    np.testing.assert_array_equal(e_idx, exact.argmin(1))
+    np.testing.assert_allclose(e_dist, exact.min(1), rtol=1e-12)
+    kk = -(-k // 256) + 11
+    u = 2.0**-24
+    gamma = kk * u / (1 - kk * u)
+    for mode in (0, 1):
+        got = oracle_mod.k5_distances(refs, q, w32, mode).astype(np.float64)
+        assert np.all(np.abs(got - exact) <= gamma * exact)
+    bad_drop = oracle_mod.k5_distances(refs, q).astype(np.float64)
+    assert np.mean(np.abs(bad_drop - exact) > gamma * exact) > 0.99
+    bad_cols = oracle_mod.k5_distances(refs, q, np.roll(w32, 1), 0).astype(np.float64)
+    assert np.mean(np.abs(bad_cols - exact) > gamma * exact) > 0.99
+
+
+def test_weighted_k5_argmin_outside_tie_band(oracle_mod):
+    """Both contraction modes return the binary64 weighted argmin wherever
+    the binary64 gap between the two nearest rows exceeds the tie band."""
+    for seed in range(4):
+        refs, q, mult, invp, w64, w32 = _weighted_case(40 + seed, r=300, k=900, b=40)
+        exact = (w64[None, None, :] * np.abs(q.astype(np.float64)[:, None, :]
+                                             - refs.astype(np.float64)[None, :, :])).sum(2)
+        srt = np.sort(exact, axis=1)
+        kk = -(-refs.shape[1] // 256) + 11
+        gamma = kk * 2.0**-24 / (1 - kk * 2.0**-24)
+        clear = (srt[:, 1] - srt[:, 0]) > 2 * gamma * srt[:, 1]
+        for mode in (0, 1):
+            idx, _ = oracle_mod.fp32_argmin(oracle_mod.k5_distances(refs, q, w32, mode))
+            np.testing.assert_array_equal(idx[clear], exact.argmin(1)[clear])
+
+
+def test_weighted_l1_closed_form(oracle_mod):
+    """Weights (10, 1, 0.01) on rows e0, 3 e1, 50 e2 and the zero query give
+    10, 3, fl(0.01f * 50): the weighted argmin is row 2, while unweighted
+    (1, 3, 50) or reversed weights (0.01, 3, 500) pick row 0."""
+    refs = np.array([[1, 0, 0], [0, 3, 0], [0, 0, 50]], F32)
+    q = np.zeros((1, 3), F32)
+    w = np.array([10, 1, 0.01], F32)
+    for mode in (0, 1):
+        d = oracle_mod.k5_distances(refs, q, w, mode)[0]
+        assert d[0] == F32(10) and d[1] == F32(3) and d[2] == F32(w[2] * F32(50))
+        assert oracle_mod.fp32_argmin(oracle_mod.k5_distances(refs, q, w, mode))[0][0] == 2
+    assert oracle_mod.fp32_argmin(oracle_mod.k5_distances(refs, q))[0][0] == 0
+    assert oracle_mod.fp32_argmin(oracle_mod.k5_distances(refs, q, w[::-1].copy(), 0))[0][0] == 0
+    cols = np.arange(3, dtype=np.uint64)
+    mult = np.array([10, 1, 1], np.uint64)
+    invp = np.array([1.0, 1.0, 0.01])
+    idx, dist = oracle_mod.weighted_l1_fp64_scan(refs, q, cols, mult, invp)
+    assert idx[0] == 2 and abs(dist[0] - 0.5) < 1e-15

@@ -98,9 +98,10 @@ Scan::Scan(int metric_value, const float* host_refs, uint64_t rows_value, uint64
                       "cannot copy representatives to the HIP device");
         }
     }
-    if (dim != 0 && dim <= kern::scan_pairs_max_dim()) {
+    if (dim != 0 && dim <= kern::scan_pairs_max_dim() && rows < kern::kPairsMaxRows) {
         // Short rows also get a transposed copy (dim x rows) for the
-        // pair-per-thread scan, whose loads are then contiguous across a wave.
+        // pair-per-thread scan, whose loads are then contiguous across a wave
+        // (that scan takes fewer than 2^25 rows, so larger sets skip it).
         std::vector<float> t(static_cast<std::size_t>(rows * dim));
         for (uint64_t r = 0; r < rows; ++r) {
             for (uint64_t c = 0; c < dim; ++c) t[c * rows + r] = host_refs[r * dim + c];
@@ -200,7 +201,7 @@ void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_
         for (const kern::ScanTile& t : kern::scan_tiles()) {
             if (t.rpt == 1 || aligned4) cands.push_back(t);
         }
-        if (scan.refs_t.bytes() != 0 && scan.dim != 0) {
+        if (scan.refs_t.bytes() != 0 && scan.dim != 0 && scan.rows < kern::kPairsMaxRows) {
             for (int tq : {1, 2, 4}) cands.push_back(kern::ScanTile{0, tq, 0, 0});
         }
         if (scan.dim != 0 && scan.dim <= kern::scan_short_max_dim()) cands.push_back(kern::ScanTile{0, 0, 0, 0});

@@ -324,11 +324,17 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     if (rows >= kPairsMaxRows) throw std::length_error("pair scan supports fewer than 2^25 reference rows");
     const size_t ldqs = (static_cast<size_t>(dim) + 3) & ~static_cast<size_t>(3);
     const size_t lds = (static_cast<size_t>(tq) * ldqs + (weighted ? ldqs : 0)) * sizeof(float);
-    const dim3 grid((rows + 255) / 256, (nq + tq - 1) / tq);
-    if (grid.y > 65535U) throw std::length_error("pair scan batch exceeds the grid limit");
+    if (tq != 1 && tq != 2 && tq != 4) throw std::invalid_argument("pair scan supports 1, 2 or 4 queries per block");
+    // grid.y carries the query tile; batches beyond 65535 tiles go in slices.
+    const uint32_t slice = 65535U * static_cast<uint32_t>(tq);
+    for (uint32_t q0 = 0; q0 < nq; q0 += slice) {
+        const uint32_t n = nq - q0 < slice ? nq - q0 : slice;
+        const float* qs = queries + static_cast<uint64_t>(q0) * ld_q;
+        unsigned long long* ks = keys + q0;
+        const dim3 grid((rows + 255) / 256, (n + tq - 1) / tq);
 #define UHANN_PAIRS(M, W, F, Q)                                                                   \
     hipLaunchKernelGGL((scan_pairs_kernel<M, W, F, Q>), grid, dim3(256), lds, stream, refs_t, ld_t, \
-                       rows, queries, ld_q, nq, weights, dim, keys)
+                       rows, qs, ld_q, n, weights, dim, ks)
     const int m = metric == Metric::l2 ? 0 : 1;
     if (tq == 1) {
         if (m == 0) UHANN_PAIRS(0, false, false, 1);
@@ -345,11 +351,10 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
         else if (!weighted) UHANN_PAIRS(1, false, false, 4);
         else if (weight_fma) UHANN_PAIRS(1, true, true, 4);
         else UHANN_PAIRS(1, true, false, 4);
-    } else {
-        throw std::invalid_argument("pair scan supports 1, 2 or 4 queries per block");
     }
 #undef UHANN_PAIRS
     check_launch("pair scan kernel");
+    }
 }
 
 
