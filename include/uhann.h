@@ -295,6 +295,59 @@ int uhann_index_query_device_async(const uhann_index* index, uhann_workspace* ws
  * coordinate, OVERFLOW for a non-finite projection or median. */
 int uhann_workspace_status(uhann_workspace* ws);
 
+/* ==== stand-alone hierarchical components ================================
+ * The query projection and the median-L1 scan as separate handles, for
+ * callers that compose them as the reference's hierarchical facades compose
+ * its detail classes. Workspaces are uhann_workspace handles created by the
+ * component's own call (a workspace from another handle is rejected). */
+typedef struct uhann_projection uhann_projection;
+typedef struct uhann_median uhann_median;
+
+/* p x k binary32 projection matrix (row-major) copied to `device`. Replaces
+ * detail::CudaFloatProjection(projection_matrix, device)
+ * (src/core/cuda_float_projection.hpp). p == 0 -> INVALID_ARGUMENT; a
+ * non-finite entry -> OVERFLOW (validate_finite_result). */
+int uhann_projection_create(const float* host_matrix, uint64_t p, uint64_t k, int device, uhann_projection** out);
+void uhann_projection_destroy(uhann_projection* projection);
+int uhann_projection_workspace_create(const uhann_projection* projection, uint64_t max_batch,
+                                      uhann_workspace** out);
+/* count x k host sampled queries -> *device_projected, a row-major
+ * count x p binary32 matrix on the device in the reference's K7 order
+ * (cuda_float_projection.cu:234-263), valid until the workspace is used
+ * again. Synchronous. Workspace owner, capacity and finite queries are
+ * checked first (INVALID_ARGUMENT); a non-finite result -> OVERFLOW.
+ * Replaces project_batch_float (both reference strategies: the binary32 K7
+ * order is the one this engine reproduces). */
+int uhann_projection_project(const uhann_projection* projection, uhann_workspace* ws, const float* host_queries,
+                             uint64_t count, const float** device_projected);
+int uhann_projection_info(const uhann_projection* projection, uint64_t* projection_dim, uint64_t* sampled_dim,
+                          uint64_t* payload_bytes, int* device);
+
+/* rows x p binary32 projected references (row-major) on `device`. Replaces
+ * detail::CudaMedianL1ScanIndex(projected_representatives, device)
+ * (src/hnsvw25/l1/hierarchical/cuda_median_l1_scan_index.hpp). rows == 0 or
+ * p == 0 or p above the supported maximum -> INVALID_ARGUMENT; a non-finite
+ * value -> OVERFLOW. */
+int uhann_median_create(const float* host_projected_refs, uint64_t rows, uint64_t p, int device, uhann_median** out);
+void uhann_median_destroy(uhann_median* median);
+int uhann_median_workspace_create(const uhann_median* median, uint64_t max_batch, uhann_workspace** out);
+/* Per query (count x p device rows, stride p) the row with the smallest
+ * median of |differences| (lowest row on ties) into host out_idx.
+ * Synchronous; a non-finite median -> OVERFLOW. Replaces
+ * query_device_batch (cuda_median_l1_scan_index.cu:429-505). */
+int uhann_median_query_device_batch(const uhann_median* median, uhann_workspace* ws, const float* device_projected,
+                                    uint64_t count, uint64_t* out_idx);
+int uhann_median_info(const uhann_median* median, uint64_t* rows, uint64_t* projection_dim, uint64_t* payload_bytes,
+                      int* device);
+
+/* Device metadata for provenance. Replaces cuda_runtime_info(device)
+ * (include/ultrahigh_ann/core/cuda_runtime_info.hpp): name, architecture
+ * major/minor (gfx950: 9, 5), total memory, the HIP version this library
+ * was built with, the runtime and driver versions. No device -> RUNTIME. */
+int uhann_device_properties(int device, char* name, size_t name_cap, int* arch_major, int* arch_minor,
+                            uint64_t* total_memory_bytes, int* compiled_runtime_version, int* runtime_version,
+                            int* driver_version);
+
 /* ==== host index construction (reference <random> order) =================
  * A std::mt19937_64 held behind a handle; the draws match the reference's
  * libstdc++ distributions call for call. */

@@ -55,6 +55,17 @@ inline void select_device(int device)
     check_hip(hipSetDevice(device), "cannot select HIP device");
 }
 
+// The device's marketing name, or "AMD GPU <architecture>" when the
+// runtime has no name for it (a missing amdgpu.ids table leaves it empty).
+inline std::string device_label(const hipDeviceProp_t& prop)
+{
+    if (prop.name[0] != '\0') return std::string(prop.name);
+    std::string arch(prop.gcnArchName);
+    const std::size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    return "AMD GPU " + (arch.empty() ? std::string("(unnamed)") : arch);
+}
+
 template <class T> class DeviceBuffer {
   public:
     DeviceBuffer() = default;

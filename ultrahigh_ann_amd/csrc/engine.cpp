@@ -60,7 +60,7 @@ Scan::Scan(int metric_value, const float* host_refs, uint64_t rows_value, uint64
     select_device(device);
     hipDeviceProp_t prop{};
     check_hip(hipGetDeviceProperties(&prop, device), "cannot query HIP device properties");
-    device_name = prop.name;
+    device_name = device_label(prop);
     if (rows == 0) {
         throw std::invalid_argument("dense scan expects at least one representative");
     }
@@ -776,7 +776,8 @@ int uhann_device_name(int device, char* buf, size_t cap)
         hipDeviceProp_t prop{};
         check_hip(hipGetDeviceProperties(&prop, device), "cannot query HIP device properties");
         if (buf == nullptr || cap == 0) throw std::invalid_argument("name buffer is empty");
-        std::strncpy(buf, prop.name, cap - 1);
+        const std::string label = device_label(prop);
+        std::strncpy(buf, label.c_str(), cap - 1);
         buf[cap - 1] = '\0';
     });
 }
