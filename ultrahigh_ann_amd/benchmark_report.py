@@ -19,9 +19,9 @@ What maps to what:
   * runs with backend=cuda run on this engine (the drop-in for the reference's
     CUDA path): exact / flat / uniform / hierarchical, L1 or L2. Strategy
     "direct" is the binary32 path in the reference's order; "gemm" is the
-    BF16 Gram with certified exact re-rank for exact and flat/uniform L2 and
-    the BF16 MFMA projection for hierarchical L2 (hierarchical L1 stays
-    binary32). backend=cpu runs are the reference's CPU path, which this
+    certified INT8 fixed-point Gram with exact re-rank for exact and
+    flat/uniform L2 and the BF16 MFMA projection for hierarchical L2
+    (hierarchical L1 stays binary32). backend=cpu runs are the reference's CPU path, which this
     engine does not ship: a setup with one is rejected before any work;
   * measurements: per batch size (clamped to the query count) the warmups and
     timed trials of a full pass over the queries in batches, host queries in,
@@ -421,7 +421,7 @@ def _strategy(uh, run: Run, metric_l1: bool) -> int:
         return uh.STRATEGY_DIRECT
     if run.index == "hierarchical":
         return uh.STRATEGY_DIRECT if metric_l1 else uh.STRATEGY_PROJECTION_BF16
-    return uh.STRATEGY_GRAM_BF16
+    return uh.STRATEGY_GRAM_INT8
 
 
 def _build_index(uh, s: Setup, run: Run, data: Dataset, probs, uniform):
@@ -440,7 +440,7 @@ def _build_index(uh, s: Setup, run: Run, data: Dataset, probs, uniform):
             cls = uh.HipSampledCoordinateL1AnnIndex if l1 else uh.HipSampledCoordinateL2AnnIndex
             index = cls(data.refs, sample, s.device)
     if run.strategy == "gemm" and run.index != "hierarchical":
-        index.enable_mixed(uh.STRATEGY_GRAM_BF16)  # the BF16 projection needs no preparation
+        index.enable_mixed(uh.STRATEGY_GRAM_INT8)  # the BF16 projection needs no preparation
     return index
 
 
