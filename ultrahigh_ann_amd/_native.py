@@ -166,6 +166,18 @@ def _declare(lib):
         "uhann_int8_rows": (c_int, [c_void_p, c_uint64, c_uint64, c_uint64, c_void_p, c_void_p, c_void_p]),
         "uhann_scan_query_device_noalloc": (c_int, [c_void_p, c_void_p, c_uint64, c_uint64, c_void_p, c_void_p,
                                                     c_void_p]),
+        "uhann_projection_create": (c_int, [f32p, c_uint64, c_uint64, c_int, POINTER(c_void_p)]),
+        "uhann_projection_destroy": (None, [c_void_p]),
+        "uhann_projection_workspace_create": (c_int, [c_void_p, c_uint64, POINTER(c_void_p)]),
+        "uhann_projection_project": (c_int, [c_void_p, c_void_p, f32p, c_uint64, POINTER(c_void_p)]),
+        "uhann_projection_info": (c_int, [c_void_p, u64p, u64p, u64p, POINTER(c_int)]),
+        "uhann_median_create": (c_int, [f32p, c_uint64, c_uint64, c_int, POINTER(c_void_p)]),
+        "uhann_median_destroy": (None, [c_void_p]),
+        "uhann_median_workspace_create": (c_int, [c_void_p, c_uint64, POINTER(c_void_p)]),
+        "uhann_median_query_device_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_uint64, u64p]),
+        "uhann_median_info": (c_int, [c_void_p, u64p, u64p, u64p, POINTER(c_int)]),
+        "uhann_device_properties": (c_int, [c_int, c_char_p, ctypes.c_size_t, POINTER(c_int), POINTER(c_int), u64p,
+                                            POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "uhann_server_create": (c_int, [POINTER(ServerConfig), POINTER(c_void_p)]),
         "uhann_server_destroy": (None, [c_void_p]),
         "uhann_server_register": (c_int, [c_void_p, POINTER(IndexSpec), u64p]),

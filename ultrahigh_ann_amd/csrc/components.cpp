@@ -147,10 +147,12 @@ int uhann_median_create(const float* host_projected_refs, uint64_t rows, uint64_
         *out = nullptr;
         if (rows == 0) throw std::invalid_argument("CUDA median L1 scan expects at least one representative");
         if (p == 0) throw std::invalid_argument("CUDA median L1 projection dimension must be positive");
-        if (p > kern::median_max_dimension()) {
-            throw std::invalid_argument("median L1 projection dimension exceeds the supported maximum");
-        }
         if (rows > 0x7fffffffULL) throw std::length_error("median L1 scan supports fewer than 2^31 rows");
+        select_device(device);
+        if (!kern::median_supported(p)) {
+            throw std::invalid_argument(
+                "CUDA median L1 projection dimension exceeds the device's per-block shared-memory capacity");
+        }
         const std::size_t n = checked_mul(rows, p, "projected representatives");
         if (host_projected_refs == nullptr) throw std::invalid_argument("projected representatives must not be null");
         for (std::size_t i = 0; i < n; ++i) {

@@ -45,6 +45,13 @@ using body::kProjQ;
 using body::kProjThreads;
 using body::kProjWaves;
 
+void check_hip_attr(hipError_t e)
+{
+    if (e != hipSuccess) {
+        throw std::runtime_error(std::string("cannot raise the kernel's LDS limit: ") + hipGetErrorString(e));
+    }
+}
+
 void check_launch(const char* what)
 {
     const hipError_t e = hipGetLastError();
@@ -82,6 +89,70 @@ median_l1_kernel(const float* __restrict__ refs_t, uint32_t rows, const float* _
     body::median_block<PMAX, EXACT>(refs_t, rows, yq, ld_y, nq, p, keys, invalid, blockIdx.y, blockIdx.x);
 }
 
+// Median-L1 for projection dimensions above the register networks (p > 128):
+// one block per (query, run of kWideRefs references); for each reference the
+// p absolute differences are padded with +inf to P2 = nextpow2(p) values in
+// dynamic LDS (4 P2 bytes, the reference's per-block requirement) and sorted
+// there with a bitonic network; the median is read as in the register
+// kernels. Query values are read from global memory (each block reads its
+// query row once per reference, from cache after the first).
+constexpr int kWideThreads = 256;
+constexpr uint32_t kWideRefs = 16;
+
+__global__ void __launch_bounds__(kWideThreads)
+median_l1_wide_kernel(const float* __restrict__ refs_t, uint32_t rows, const float* __restrict__ yq, uint64_t ld_y,
+                      uint32_t nq, uint32_t p, uint32_t p2, unsigned long long* __restrict__ keys, int* invalid)
+{
+    extern __shared__ float s[];
+    const uint32_t q = blockIdx.y;
+    if (q >= nq) return;
+    const uint32_t t = threadIdx.x;
+    const float* qrow = yq + static_cast<uint64_t>(q) * ld_y;
+    const uint32_t r0 = blockIdx.x * kWideRefs;
+    const uint32_t r1 = min(rows, r0 + kWideRefs);
+    unsigned long long best = ~0ULL;
+    int bad = 0;
+    for (uint32_t r = r0; r < r1; ++r) {
+        for (uint32_t c = t; c < p2; c += kWideThreads) {
+            float d = __builtin_inff();
+            if (c < p) {
+                d = __builtin_fabsf(refs_t[static_cast<uint64_t>(c) * rows + r] - qrow[c]);
+                if (!__builtin_isfinite(d)) {
+                    bad = 1;
+                    d = __builtin_inff();
+                }
+            }
+            s[c] = d;
+        }
+        __syncthreads();
+        for (uint32_t size = 2; size <= p2; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t i = t; i < p2 / 2; i += kWideThreads) {
+                    const uint32_t a = 2 * stride * (i / stride) + (i % stride);
+                    const uint32_t b = a + stride;
+                    const float x = s[a];
+                    const float y = s[b];
+                    const bool up = (a & size) == 0;
+                    s[a] = up ? fminf(x, y) : fmaxf(x, y);
+                    s[b] = up ? fmaxf(x, y) : fminf(x, y);
+                }
+                __syncthreads();
+            }
+        }
+        if (t == 0) {
+            const uint32_t mid = p / 2;
+            float med = s[mid];
+            if (p % 2 == 0) med = s[mid - 1] + (med - s[mid - 1]) * 0.5F;
+            if (!__builtin_isfinite(med)) bad = 1;
+            const unsigned long long key = (static_cast<unsigned long long>(__float_as_uint(med)) << 32) | r;
+            best = key < best ? key : best;
+        }
+        __syncthreads();
+    }
+    if (bad) atomicExch(invalid, 1);
+    if (t == 0 && best != ~0ULL) atomicMin(keys + q, best);
+}
+
 __global__ void zero_results_kernel(uint64_t* idx, float* dist, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -105,7 +176,26 @@ void project_fp32(const float* x, uint64_t ld_x, uint32_t nq, uint32_t k, const 
     check_launch("binary32 projection kernel");
 }
 
-uint32_t median_max_dimension() { return 128; }
+uint32_t median_register_dimension() { return 128; }
+
+uint32_t next_pow2(uint32_t v)
+{
+    uint32_t p2 = 1;
+    while (p2 < v) p2 <<= 1;
+    return p2;
+}
+
+// The reference's rule (cuda_median_l1_scan_index.cu:314-336): the padded
+// sort of 4 nextpow2(p) bytes must fit the device's per-block shared memory.
+bool median_supported(uint64_t p)
+{
+    if (p == 0 || p > (1ULL << 30)) return false;
+    if (p <= median_register_dimension()) return true;
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+    return static_cast<uint64_t>(next_pow2(static_cast<uint32_t>(p))) * sizeof(float) <= prop.sharedMemPerBlock;
+}
 uint64_t project_bf16_rows(uint64_t p) { return (p + 31U) / 32U * 32U; }
 uint64_t project_bf16_ld(uint64_t k) { return (k + 15U) / 16U * 16U + 16U; }
 
@@ -124,8 +214,24 @@ void median_l1(const float* refs_t, uint32_t rows, const float* yq, uint64_t ld_
                uint32_t p, unsigned long long* keys, int* invalid, hipStream_t stream)
 {
     if (nq == 0 || rows == 0) return;
-    if (p == 0 || p > median_max_dimension()) {
-        throw std::invalid_argument("median-L1 projection dimension must be in [1, 128]");
+    if (!median_supported(p)) {
+        throw std::invalid_argument("median-L1 projection dimension exceeds the device's per-block LDS for its sort");
+    }
+    if (p > median_register_dimension()) {
+        const uint32_t p2 = next_pow2(p);
+        const size_t lds = static_cast<size_t>(p2) * sizeof(float);
+        if (lds > 65536) {
+            check_hip_attr(hipFuncSetAttribute(reinterpret_cast<const void*>(median_l1_wide_kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        }
+        for (uint32_t q0 = 0; q0 < nq; q0 += 65535U) {
+            const uint32_t n = min(65535U, nq - q0);
+            const dim3 grid((rows + kWideRefs - 1) / kWideRefs, n);
+            median_l1_wide_kernel<<<grid, kWideThreads, lds, stream>>>(refs_t, rows, yq + static_cast<uint64_t>(q0) * ld_y,
+                                                                       ld_y, n, p, p2, keys + q0, invalid);
+            check_launch("wide median-L1 kernel");
+        }
+        return;
     }
     // grid.y carries the query; batches above 65535 go in slices.
     for (uint32_t q0 = 0; q0 < nq; q0 += 65535U) {

@@ -26,10 +26,14 @@ uint64_t project_bf16_ld(uint64_t k);
 // Median-L1 scan: refs_t is p x rows (transposed projected references), yq
 // nq x p (row stride ld_y). keys[q] receives atomicMin((median bits << 32) |
 // row) and must hold ~0 before the launch. *invalid = 1 on a non-finite
-// difference or median. p must be in [1, median_max_dimension()].
+// difference or median. p up to median_register_dimension() sorts in
+// registers; larger p (median_supported: 4 nextpow2(p) bytes within the
+// current device's per-block shared memory) sorts in LDS.
 void median_l1(const float* refs_t, uint32_t rows, const float* yq, uint64_t ld_y, uint32_t nq,
                uint32_t p, unsigned long long* keys, int* invalid, hipStream_t stream);
-uint32_t median_max_dimension();
+uint32_t median_register_dimension();
+bool median_supported(uint64_t p);
+uint32_t next_pow2(uint32_t v);
 
 // idx[i] = 0, dist[i] = 0 for i < n (either may be null).
 void zero_results(uint64_t* idx, float* dist, uint32_t n, hipStream_t stream);

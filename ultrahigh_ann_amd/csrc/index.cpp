@@ -274,8 +274,12 @@ int uhann_hierarchical_create(int metric, const float* host_refs, uint64_t rows,
         ix->p = projection_dim;
         ix->p_ld = padded_ld(projection_dim);
         if (ix->k != 0 && projection == nullptr) throw std::invalid_argument("projection must not be null");
-        if (metric == UHANN_METRIC_L1 && projection_dim > kern::median_max_dimension()) {
-            throw std::invalid_argument("median-L1 projection dimension above 128 is not supported");
+        if (metric == UHANN_METRIC_L1) {
+            select_device(device);
+            if (!kern::median_supported(projection_dim)) {
+                throw std::invalid_argument(
+                    "CUDA median L1 projection dimension exceeds the device's per-block shared-memory capacity");
+            }
         }
         std::vector<float> proj32;
         std::vector<float> projected;

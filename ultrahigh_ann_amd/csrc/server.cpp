@@ -802,6 +802,9 @@ int uhann_server_register(uhann_server* s, const uhann_index_spec* spec, uint64_
             const uint64_t n = checked_mul(sp.projection_dim, sp.sample_count, "projection");
             if (n != 0 && sp.projection == nullptr) throw std::invalid_argument("projection must not be null");
             e->projection.assign(sp.projection, sp.projection + n);
+            if (sp.metric == UHANN_METRIC_L1 && sp.projection_dim > kern::median_register_dimension()) {
+                throw std::invalid_argument("serving supports median-L1 projection dimensions up to 128");
+            }
         }
         e->spec.sample = nullptr;
         e->spec.projection = nullptr;
