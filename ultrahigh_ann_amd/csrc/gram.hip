@@ -74,6 +74,32 @@ __device__ __forceinline__ double block_sum_f64(double v, double* scratch)
     return total;
 }
 
+// Four block sums at once (one pair of barriers), in the same order as
+// block_sum_f64: a shuffle tree per wave, then the waves in order.
+__device__ __forceinline__ double4 block_sum4_f64(double4 v, double4* scratch)
+{
+    for (int off = 32; off >= 1; off /= 2) {
+        v.x += __shfl_down(v.x, off);
+        v.y += __shfl_down(v.y, off);
+        v.z += __shfl_down(v.z, off);
+        v.w += __shfl_down(v.w, off);
+    }
+    const int lane = threadIdx.x % 64;
+    const int wave = threadIdx.x / 64;
+    __syncthreads();
+    if (lane == 0) scratch[wave] = v;
+    __syncthreads();
+    double4 t = make_double4(0.0, 0.0, 0.0, 0.0);
+    const int waves = blockDim.x / 64;
+    for (int w = 0; w < waves; ++w) {
+        t.x += scratch[w].x;
+        t.y += scratch[w].y;
+        t.z += scratch[w].z;
+        t.w += scratch[w].w;
+    }
+    return t;
+}
+
 // One block of 256 threads per row.
 __global__ void __launch_bounds__(256)
 to_bf16_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows, uint32_t dim,
@@ -1209,107 +1235,110 @@ constexpr int kI8Code = 32512;           // |n| <= 127 * 256
 // 2^-24 of itself, less than the 2^-20 added first).
 __device__ __forceinline__ float round_up_f(double v) { return static_cast<float>(v * (1.0 + 0x1p-20)); }
 
-// One block of 256 threads per row: max |x|, then codes, bytes and sums.
-__global__ void __launch_bounds__(256)
+// One block of 1024 threads per row, the row held in registers: thread t
+// holds the four-column groups t + 1024 i (i < NPT, 4096 NPT >= the padded
+// row), so the row is read once for max |x| and then turned into codes,
+// bytes and binary64 sums. Groups past the row's end are zero (padding).
+template <int NPT>
+__global__ void __launch_bounds__(1024)
 to_i8x2_rows_kernel(const float* __restrict__ src, uint64_t ld_src, uint32_t rows, uint32_t dim,
                     uint8_t* __restrict__ dst, uint64_t ld_dst, I8Stats* __restrict__ stats)
 {
-    __shared__ double scratch[8];
-    __shared__ float fscratch[8];
+    __shared__ double4 scratch4[16];
+    __shared__ float fscratch[16];
     const uint32_t row = blockIdx.x;
     if (row >= rows) return;
     const float* s = src + static_cast<uint64_t>(row) * ld_src;
     uint8_t* o = dst + static_cast<uint64_t>(row) * ld_dst;
     const bool vec = ld_src % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15U) == 0;
-    const uint32_t full = vec ? dim / 4 * 4 : 0;
+    const uint32_t pdim = static_cast<uint32_t>(ld_dst / 2);
+    float x[NPT][4];
     float mx = 0.0F;
     int bad = 0;
-    {
-        auto see = [&](float x) {
-            mx = fmaxf(mx, fabsf(x));
-            bad |= !isfinite(x);
-        };
-        const uint32_t step = blockDim.x * 4;
-        uint32_t c = threadIdx.x * 4;
-        for (; c + 3 * step < full; c += 4 * step) {
-            float4 x[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const float4*>(s + c + u * step);
+    for (int i = 0; i < NPT; ++i) {
+        const uint32_t c = (threadIdx.x + 1024U * i) * 4U;
+        if (vec && c + 3 < dim) {
+            const float4 v = *reinterpret_cast<const float4*>(s + c);
+            x[i][0] = v.x;
+            x[i][1] = v.y;
+            x[i][2] = v.z;
+            x[i][3] = v.w;
+        } else {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                see(x[u].x);
-                see(x[u].y);
-                see(x[u].z);
-                see(x[u].w);
-            }
+            for (int u = 0; u < 4; ++u) x[i][u] = c + u < dim ? s[c + u] : 0.0F;
         }
-        for (; c < full; c += step) {
-            const float4 x = *reinterpret_cast<const float4*>(s + c);
-            see(x.x);
-            see(x.y);
-            see(x.z);
-            see(x.w);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            mx = fmaxf(mx, fabsf(x[i][u]));
+            bad |= !isfinite(x[i][u]);
         }
-        for (uint32_t e = full + threadIdx.x; e < dim; e += blockDim.x) see(s[e]);
     }
     mx = block_max_f32(mx, fscratch);
     const bool finite = __syncthreads_or(bad) == 0;
     const float scale = !finite ? __builtin_nanf("") : (mx > 0.0F ? mx / 127.0F : 1.0F);
     const float inv = (finite && mx > 0.0F) ? static_cast<float>(kI8Code) / mx : 0.0F;
     const double sd = static_cast<double>(scale) * 0.00390625;  // s / 256, exact
-    double n2 = 0.0, e2 = 0.0, h2 = 0.0, b2 = 0.0;
-    // Columns in groups of four (padding columns are zero): four a bytes and
-    // four b bytes per group, written as two 32-bit words.
-    const uint32_t pdim = static_cast<uint32_t>(ld_dst / 2);
-    for (uint32_t c = threadIdx.x * 4; c < pdim; c += blockDim.x * 4) {
-        float x[4];
-        if (vec && c + 3 < dim) {
-            const float4 v = *reinterpret_cast<const float4*>(s + c);
-            x[0] = v.x;
-            x[1] = v.y;
-            x[2] = v.z;
-            x[3] = v.w;
-        } else {
+    if (!finite) {
+        // all codes zero (the statistics below are NaN)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = c + u < dim ? s[c + u] : 0.0F;
-        }
-        uint32_t pa = 0, pb = 0;
+        for (int i = 0; i < NPT; ++i)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[i][u] = 0.0F;
+    }
+    // n2 = sum x^2 and e2 = sum (x - x^)^2 in binary64 (fused multiply-adds,
+    // one rounding per term); the code sums sum n^2 (< 2^53, so exact in
+    // binary64) and sum b^2 (< 2^31 per thread) are integers, and
+    // ||x^||^2 = (s/256)^2 sum n^2 is formed once at the end.
+    double n2 = 0.0, e2 = 0.0, c2 = 0.0;
+    uint32_t b2 = 0;
+    // Four a bytes and four b bytes per group, written as two 32-bit words.
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const uint32_t c = (threadIdx.x + 1024U * i) * 4U;
+        if (c >= pdim) break;
+        // m = n + 32896 lies in [384, 65408]: its byte 1 is a + 128 and its
+        // byte 0 is b + 128 (a = floor((n + 128) / 256) in [-127, 127],
+        // b = n - 256 a in [-128, 127]); flipping bit 7 turns each into the
+        // two's-complement byte.
+        uint32_t m[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            int n = 0;
-            if (finite) {
-                n = static_cast<int>(rintf(x[u] * inv));
-                n = n < -kI8Code ? -kI8Code : (n > kI8Code ? kI8Code : n);
-            }
-            const int a = (n + 128) >> 8;  // floor((n + 128) / 256): a in [-127, 127]
-            const int b = n - a * 256;     // b in [-128, 127]
-            const double xd = x[u];
-            const double hd = sd * static_cast<double>(n);  // exact: 24-bit s times 16-bit n
-            const double ed = xd - hd;
-            n2 += xd * xd;
-            e2 += ed * ed;
-            h2 += hd * hd;
-            b2 += static_cast<double>(b * b);
-            pa |= (static_cast<uint32_t>(a) & 255U) << (8 * u);
-            pb |= (static_cast<uint32_t>(b) & 255U) << (8 * u);
+            int n = static_cast<int>(rintf(x[i][u] * inv));
+            n = n < -kI8Code ? -kI8Code : (n > kI8Code ? kI8Code : n);
+            m[u] = static_cast<uint32_t>(n + 32896);
+            const int b = static_cast<int>(m[u] & 255U) - 128;
+            const double xd = x[i][u];
+            const double nd = static_cast<double>(n);
+            const double ed = fma(-sd, nd, xd);  // x - s n / 256, the product exact (24 x 16 bits)
+            n2 = fma(xd, xd, n2);
+            e2 = fma(ed, ed, e2);
+            c2 = fma(nd, nd, c2);  // integers below 2^53: exact
+            b2 += static_cast<uint32_t>(b * b);
         }
+        // bytes 0..3 of t01 are m0.b0, m0.b1, m1.b0, m1.b1; of t23 likewise
+        // for m2, m3; the selector picks bytes of {t23, t01} (t01 low)
+        const uint32_t t01 = m[0] | (m[1] << 16);
+        const uint32_t t23 = m[2] | (m[3] << 16);
+        const uint32_t pa = __builtin_amdgcn_perm(t23, t01, 0x07050301U) ^ 0x80808080U;
+        const uint32_t pb = __builtin_amdgcn_perm(t23, t01, 0x06040200U) ^ 0x80808080U;
         uint8_t* chunk = o + (c / 64U) * 128U + (c % 64U);
         *reinterpret_cast<uint32_t*>(chunk) = pa;
         *reinterpret_cast<uint32_t*>(chunk + 64) = pb;
     }
-    n2 = block_sum_f64(n2, scratch);
-    e2 = block_sum_f64(e2, scratch);
-    h2 = block_sum_f64(h2, scratch);
-    b2 = block_sum_f64(b2, scratch);
+    double4 sums = block_sum4_f64(make_double4(n2, e2, c2, static_cast<double>(b2)), scratch4);
     if (threadIdx.x == 0) {
         I8Stats st{};
         if (finite) {
-            st.norm2 = n2;
+            // round_up_f's 2^-20 covers the one rounding of h2 and of each
+            // square root
+            const double h2 = (sd * sd) * sums.z;
+            st.norm2 = sums.x;
             st.scale = scale;
-            st.err = round_up_f(sqrt(e2));
+            st.err = round_up_f(sqrt(sums.y));
             st.hat = round_up_f(sqrt(h2));
-            st.lo = round_up_f(sd * sqrt(b2));
-            st.norm = round_up_f(sqrt(n2));
+            st.lo = round_up_f(sd * sqrt(sums.w));
+            st.norm = round_up_f(sqrt(sums.x));
         } else {
             const float nan = __builtin_nanf("");
             st.norm2 = static_cast<double>(nan);
@@ -1421,8 +1450,10 @@ gram_i8_fixup_kernel(GramSplit sp, uint32_t M, uint32_t N, const I8Stats* __rest
 // one 128-byte chunk per row (64 columns of a, then of b), double-buffered in
 // LDS with the next step's loads issued before the barrier, as in the BF16
 // kernel. Per 32 columns and output tile: one MFMA for H, two for X.
+// Four-wave blocks are bounded to two blocks per compute unit (at most 256
+// registers per lane), so one block's barrier wait overlaps the other's MFMAs.
 template <int WM, int WN, int TM, int TN>
-__global__ void __launch_bounds__(64 * WM * WN)
+__global__ void __launch_bounds__(64 * WM * WN, WM * WN == 4 ? 2 : 1)
 gram_i8x2_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const uint8_t* __restrict__ V,
                  uint64_t ldv, uint32_t N, uint32_t K, const I8Stats* __restrict__ qs,
                  const I8Stats* __restrict__ vs, float* __restrict__ out, uint64_t ldo, uint32_t tiles_m,
@@ -1823,7 +1854,20 @@ void to_i8x2_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim
 {
     if (rows == 0) return;
     if (ld_dst % kBKI8 != 0 || ld_dst / 2 < dim) throw std::invalid_argument("int8 rows need 128 ceil(dim / 64) bytes");
-    to_i8x2_rows_kernel<<<rows, 256, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    const uint64_t pdim = ld_dst / 2;
+    if (pdim > kI8MaxDim) throw std::length_error("int8 rows are limited to 65536 columns");
+    // the fewest four-column groups per thread (of 1024) that cover the padded row
+    if (pdim <= 4096) {
+        to_i8x2_rows_kernel<1><<<rows, 1024, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    } else if (pdim <= 8192) {
+        to_i8x2_rows_kernel<2><<<rows, 1024, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    } else if (pdim <= 16384) {
+        to_i8x2_rows_kernel<4><<<rows, 1024, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    } else if (pdim <= 32768) {
+        to_i8x2_rows_kernel<8><<<rows, 1024, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    } else {
+        to_i8x2_rows_kernel<16><<<rows, 1024, 0, stream>>>(src, ld_src, rows, dim, dst, ld_dst, stats);
+    }
     check_launch("int8 conversion kernel");
 }
 
@@ -1834,10 +1878,15 @@ namespace {
 //   1: 4x4 waves, 2x1 tiles (256 x 128, 16 waves)
 //   2: 2x4 waves, 2x2 tiles (128 x 256, 8 waves)
 //   3: 4x4 waves, 1x2 tiles (128 x 256, 16 waves)
+//   4: 2x2 waves, 2x2 tiles (128 x 128, 4 waves: two blocks per CU)
+// group_m: tile rows swept together (tile_origin). Each XCD runs 32
+// consecutive tiles at once; four rows of 256 x 128 tiles touch
+// 4 x 256 + 8 x 128 operand rows per K step, eight rows 8 x 256 + 4 x 128.
 struct I8Shape {
     int wm, wn, tm, tn;
+    uint32_t group_m;
 };
-constexpr I8Shape kI8Shapes[4] = {{4, 2, 2, 2}, {4, 4, 2, 1}, {2, 4, 2, 2}, {4, 4, 1, 2}};
+constexpr I8Shape kI8Shapes[5] = {{4, 2, 2, 2, 4}, {4, 4, 2, 1, 8}, {2, 4, 2, 2, 8}, {4, 4, 1, 2, 8}, {2, 2, 2, 2, 8}};
 constexpr int kI8DefaultTile = 0;
 
 }  // namespace
@@ -1870,7 +1919,7 @@ void gram_i8x2(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uin
     const uint32_t bn = 32U * sh.tn * sh.wn;
     const uint32_t tiles_m = (m + bm - 1) / bm;
     const uint32_t tiles_n = (n + bn - 1) / bn;
-    const uint32_t group_m = 8;
+    const uint32_t group_m = sh.group_m;
     GramSplit sp;
     sp.full = tiles_m * tiles_n;
     if (tile < 0) sp = split_for(tiles_m * tiles_n, k / kBKI8, static_cast<uint64_t>(2U * bm * bn), partials, partial_floats);
@@ -1890,6 +1939,7 @@ void gram_i8x2(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uin
         UHANN_I8_CASE(1, 4, 4, 2, 1)
         UHANN_I8_CASE(2, 2, 4, 2, 2)
         UHANN_I8_CASE(3, 4, 4, 1, 2)
+        UHANN_I8_CASE(4, 2, 2, 2, 2)
     default:
         break;
     }
@@ -1907,6 +1957,6 @@ void select_rerank_i8(const SelectArgs& args, hipStream_t stream)
 
 uint32_t gram_i8_k_padding() { return kBKI8; }
 uint32_t gram_i8_max_dim() { return kI8MaxDim; }
-int gram_i8_tile_count() { return 4; }
+int gram_i8_tile_count() { return 5; }
 
 }  // namespace uhann::kern
