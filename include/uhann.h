@@ -462,7 +462,7 @@ int uhann_scan_fp8_estimates(const uhann_scan* scan, uhann_workspace* ws, const 
 /* The GRAM_INT8 strategy's estimates ||q||^2 + ||v||^2 - 2 s_q s_v (H + X /
  * 256) (exact int32 sums, binary64 combination, rounded to binary32) for
  * count device queries against every reference, as uhann_scan_gram_estimates
- * (tile -1: the default shape with its K split; 0 .. 3 force a block shape,
+ * (tile -1: the default shape with its K split; 0 .. 5 force a block shape,
  * every tile whole; all give the same bits, since the sums are exact). Needs
  * uhann_scan_enable_mixed(GRAM_INT8). No counterpart in the reference: for
  * diagnostics and tests. */

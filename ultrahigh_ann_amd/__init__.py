@@ -377,7 +377,7 @@ class _DenseScan:
                        stream=None, ld_queries=None):
         """GRAM_INT8 estimates ||q||^2 + ||v||^2 - 2 s_q s_v (H + X / 256)
         into the device tensor out (count x rows, row stride out.stride(0));
-        tile -1 runs the default block shape with its K split, 0..3 force a
+        tile -1 runs the default block shape with its K split, 0..5 force a
         shape (all give the same bits: the integer sums are exact)."""
         ld = ld_queries if ld_queries is not None else self.dim
         check(

@@ -1562,6 +1562,232 @@ gram_i8x2_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const 
     i8_epilogue<TM, TN>(acc_h, acc_x, m0 + wm, n0 + wn, lane, M, N, qs, vs, out, ldo);
 }
 
+// s_waitcnt immediates (gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt in
+// 6:4, lgkmcnt in 11:8; a field at its maximum does not wait).
+constexpr int kWaitLgkm0 = 0xC07F;  // lgkmcnt(0)
+constexpr int kWaitVm6 = 0x0F76;    // vmcnt(6)
+constexpr int kWaitVm0 = 0x0F70;    // vmcnt(0)
+// Word 3 of a raw (stride 0) buffer resource: 32-bit data format, no swizzle.
+constexpr int kRawBufferWord3 = 0x00020000;
+
+constexpr int kPpBM = 256;
+constexpr int kPpBN = 128;
+constexpr int kPpStage = (kPpBM + kPpBN) * kBKI8;  // one K step of both operands: 49152 bytes
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// One K step's MFMA operands of a 64 x 64 wave tile: [sub-step][tile].
+struct PpFrags {
+    intx4 ah[2][2], al[2][2], bh[2][2], bl[2][2];
+};
+
+// Ping-pong GRAM_INT8 block, 256 x 128 outputs, 8 waves in two groups of
+// four. Wave w belongs to group w / 4 and computes the 64 x 64 outputs at rows
+// 64 (w / 2), columns 64 (w % 2) (the 4 x 2 wave grid of shape 0, so the
+// split-tile fix-up is shared). Waves w and w + 4 run on the same SIMD.
+//
+// K steps (128 bytes per row) reach LDS by direct buffer-to-LDS loads (no
+// staging registers, no LDS store instructions) into a ring of three stages,
+// two steps ahead. Within a stage a row's 16-byte chunk c sits in slot
+// c ^ (row % 8), so the MFMA fragment reads (32 rows of one chunk) hit every
+// bank once; a load instruction writes 8 consecutive rows (1 KB), its lanes
+// fetching the chunks in slot order.
+//
+// The two groups alternate, half a step apart, between
+//   memory phase (step s): read all 16 fragments of step s into registers,
+//                          issue this wave's loads of step s + 2;
+//   MFMA phase (step s):   the 24 MFMAs of step s from registers;
+// with a block barrier between phases, so on each SIMD one wave feeds the
+// matrix core while the other reads LDS. Barrier B_t separates half steps;
+// group 0 reads step s in half step 2s + 1, group 1 in 2s + 2. Step s + 2 is
+// loaded (by both groups, in their memory phases of step s) into the stage
+// that held step s - 1, free since B_{2s+1}, and each wave waits for its own
+// loads of step s + 1 (vmcnt(6): only step s + 2's six loads may remain)
+// before B_{2s+3}, the barrier after which group 0 reads step s + 1. Each
+// group passes 2S + 2 barriers in all (S steps). Steps past the last reload
+// the last step into a stage nobody reads again, so the load counts stay
+// uniform; every load has landed before the epilogue.
+__global__ void __launch_bounds__(512)
+gram_i8x2_pp_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const uint8_t* __restrict__ V,
+                    uint64_t ldv, uint32_t N, uint32_t K, const I8Stats* __restrict__ qs,
+                    const I8Stats* __restrict__ vs, float* __restrict__ out, uint64_t ldo, uint32_t tiles_m,
+                    uint32_t tiles_n, uint32_t group_m, GramSplit sp)
+{
+    __shared__ __attribute__((aligned(1024))) uint8_t st0[kPpStage];
+    __shared__ __attribute__((aligned(1024))) uint8_t st1[kPpStage];
+    __shared__ __attribute__((aligned(1024))) uint8_t st2[kPpStage];
+
+    const GramBlock gb = gram_block(blockIdx.x, sp, K, kBKI8);
+    const uint2 o = tile_origin(gb.tile, tiles_m, tiles_n, group_m);
+    const uint32_t m0 = o.x * kPpBM;
+    const uint32_t n0 = o.y * kPpBN;
+
+    const int tid = threadIdx.x;
+    const int lane = tid % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int grp = wave / 4;
+    const int lw = wave % 4;
+
+    // Loads: this wave brings query rows 128 grp + 32 lw + [0, 32) (four
+    // instructions) and reference rows 64 grp + 16 lw + [0, 16) (two); lane L
+    // fetches chunk (L % 8) ^ (L / 8) of row L / 8 of the instruction's eight.
+    const uint32_t ldq32 = static_cast<uint32_t>(ldq);
+    const uint32_t ldv32 = static_cast<uint32_t>(ldv);
+    const __amdgpu_buffer_rsrc_t rq = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t*>(Q + static_cast<uint64_t>(m0) * ldq), 0, static_cast<int>(kPpBM * ldq32),
+        kRawBufferWord3);
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t*>(V + static_cast<uint64_t>(n0) * ldv), 0, static_cast<int>(kPpBN * ldv32),
+        kRawBufferWord3);
+    const int qrow0 = 128 * grp + 32 * lw;
+    const int vrow0 = 64 * grp + 16 * lw;
+    const uint32_t chunk = static_cast<uint32_t>((lane % 8) ^ (lane / 8));
+    const uint32_t qoff = static_cast<uint32_t>(qrow0 + lane / 8) * ldq32 + 16U * chunk;
+    const uint32_t voff = static_cast<uint32_t>(vrow0 + lane / 8) * ldv32 + 16U * chunk;
+
+    // Fragments: lane l holds row l % 32 and bytes 16 (l / 32) .. + 15 of a
+    // 32-byte sub-step (logical chunk 2 kk + l / 32 of the a plane, + 4 for
+    // the b plane), as in gram_i8x2_kernel.
+    const int lrow = lane % 32;
+    const int a_base = (128 * grp + 64 * (lw / 2) + lrow) * kBKI8;
+    const int b_base = (kPpBM + 64 * (lw % 2) + lrow) * kBKI8;
+    int coff[2][2];
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) coff[kk][p] = 16 * ((2 * kk + 4 * p + lane / 32) ^ (lane % 8));
+
+    intx16 acc_h[2][2], acc_x[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc_h[i][j][v] = acc_x[i][j][v] = 0;
+
+    const uint32_t S = (gb.k_end - gb.k_begin) / kBKI8;
+    const uint32_t k_last = gb.k_end - kBKI8;
+
+    auto issue = [&](uint8_t* st, uint32_t step) {
+        const uint32_t k = min(gb.k_begin + step * kBKI8, k_last);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rq, (lds_ptr_t)(st + (qrow0 + 8 * u) * kBKI8), 16, qoff,
+                                                     k + 8U * u * ldq32, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rv, (lds_ptr_t)(st + (kPpBM + vrow0 + 8 * u) * kBKI8), 16, voff,
+                                                     k + 8U * u * ldv32, 0, 0);
+        }
+    };
+    auto read = [&](const uint8_t* st, PpFrags& f) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                f.ah[kk][i] = *reinterpret_cast<const intx4*>(st + a_base + i * 32 * kBKI8 + coff[kk][0]);
+                f.al[kk][i] = *reinterpret_cast<const intx4*>(st + a_base + i * 32 * kBKI8 + coff[kk][1]);
+                f.bh[kk][i] = *reinterpret_cast<const intx4*>(st + b_base + i * 32 * kBKI8 + coff[kk][0]);
+                f.bl[kk][i] = *reinterpret_cast<const intx4*>(st + b_base + i * 32 * kBKI8 + coff[kk][1]);
+            }
+    };
+    auto mfma = [&](const PpFrags& f) {
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc_h[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.ah[kk][i], f.bh[kk][j], acc_h[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc_x[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.ah[kk][i], f.bl[kk][j], acc_x[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc_x[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.al[kk][i], f.bh[kk][j], acc_x[i][j], 0, 0, 0);
+        }
+    };
+    // A block barrier that nothing is scheduled across (no fence: the loads'
+    // completion is awaited explicitly with s_waitcnt).
+#define UHANN_PP_SYNC()                       \
+    __builtin_amdgcn_sched_barrier(0);        \
+    asm volatile("" ::: "memory");            \
+    __builtin_amdgcn_s_barrier();             \
+    asm volatile("" ::: "memory");            \
+    __builtin_amdgcn_sched_barrier(0);
+#define UHANN_PP_G0(CUR, NXT, s)                  \
+    read(CUR, f);                                 \
+    issue(NXT, (s) + 2);                          \
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);       \
+    UHANN_PP_SYNC()                               \
+    mfma(f);                                      \
+    __builtin_amdgcn_s_waitcnt(kWaitVm6);         \
+    UHANN_PP_SYNC()
+#define UHANN_PP_G1(CUR, NXT, s)                  \
+    read(CUR, f);                                 \
+    issue(NXT, (s) + 2);                          \
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);       \
+    __builtin_amdgcn_s_waitcnt(kWaitVm6);         \
+    UHANN_PP_SYNC()                               \
+    mfma(f);                                      \
+    UHANN_PP_SYNC()
+
+    PpFrags f;
+    // Prologue: steps 0 and 1; step 0 visible to all at B_1.
+    issue(st0, 0);
+    issue(st1, 1);
+    __builtin_amdgcn_s_waitcnt(kWaitVm6);
+    UHANN_PP_SYNC()
+    uint32_t s = 0;
+    if (grp == 0) {
+        for (; s + 3 <= S; s += 3) {
+            UHANN_PP_G0(st0, st2, s)
+            UHANN_PP_G0(st1, st0, s + 1)
+            UHANN_PP_G0(st2, st1, s + 2)
+        }
+        if (s < S) {
+            UHANN_PP_G0(st0, st2, s)
+            ++s;
+        }
+        if (s < S) {
+            UHANN_PP_G0(st1, st0, s)
+            ++s;
+        }
+        UHANN_PP_SYNC()  // B_{2S+2}, group 1's last MFMA phase
+    } else {
+        UHANN_PP_SYNC()  // B_2: group 0 reads step 0 meanwhile
+        for (; s + 3 <= S; s += 3) {
+            UHANN_PP_G1(st0, st2, s)
+            UHANN_PP_G1(st1, st0, s + 1)
+            UHANN_PP_G1(st2, st1, s + 2)
+        }
+        if (s < S) {
+            UHANN_PP_G1(st0, st2, s)
+            ++s;
+        }
+        if (s < S) {
+            UHANN_PP_G1(st1, st0, s)
+            ++s;
+        }
+    }
+#undef UHANN_PP_G0
+#undef UHANN_PP_G1
+#undef UHANN_PP_SYNC
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+
+    if (gb.part >= 0) {
+        const uint64_t unit = static_cast<uint64_t>(gb.tile - sp.full) * sp.parts + gb.part;
+        store_i8_partials<512, 2, 2>(sp.partials + unit * (2 * kPpBM * kPpBN), acc_h, acc_x, tid);
+        return;
+    }
+    i8_epilogue<2, 2>(acc_h, acc_x, m0 + 64 * (wave / 2), n0 + 64 * (wave % 2), lane, M, N, qs, vs, out, ldo);
+}
+
 // Lower and upper ends of ||q - v||^2 from a GRAM_INT8 estimate e: the bound
 // of the file comment above, computed in binary64 from the rounded-up row
 // norms, plus 2^-23 |e| for the binary32 rounding of the estimate and
@@ -1879,6 +2105,7 @@ namespace {
 //   2: 2x4 waves, 2x2 tiles (128 x 256, 8 waves)
 //   3: 4x4 waves, 1x2 tiles (128 x 256, 16 waves)
 //   4: 2x2 waves, 2x2 tiles (128 x 128, 4 waves: two blocks per CU)
+//   5: ping-pong 256 x 128 (gram_i8x2_pp_kernel; the 4x2 wave grid of shape 0)
 // group_m: tile rows swept together (tile_origin). Each XCD runs 32
 // consecutive tiles at once; four rows of 256 x 128 tiles touch
 // 4 x 256 + 8 x 128 operand rows per K step, eight rows 8 x 256 + 4 x 128.
@@ -1886,8 +2113,9 @@ struct I8Shape {
     int wm, wn, tm, tn;
     uint32_t group_m;
 };
-constexpr I8Shape kI8Shapes[5] = {{4, 2, 2, 2, 4}, {4, 4, 2, 1, 8}, {2, 4, 2, 2, 8}, {4, 4, 1, 2, 8}, {2, 2, 2, 2, 8}};
-constexpr int kI8DefaultTile = 0;
+constexpr I8Shape kI8Shapes[6] = {{4, 2, 2, 2, 4}, {4, 4, 2, 1, 8}, {2, 4, 2, 2, 8},
+                                   {4, 4, 1, 2, 8}, {2, 2, 2, 2, 8}, {4, 2, 2, 2, 4}};
+constexpr int kI8DefaultTile = 5;
 
 }  // namespace
 
@@ -1940,6 +2168,14 @@ void gram_i8x2(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uin
         UHANN_I8_CASE(2, 2, 4, 2, 2)
         UHANN_I8_CASE(3, 4, 4, 1, 2)
         UHANN_I8_CASE(4, 2, 2, 2, 2)
+    case 5:
+        gram_i8x2_pp_kernel<<<grid, block, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo, tiles_m, tiles_n,
+                                                        group_m, sp);
+        if (sp.tail > 0) {
+            gram_i8_fixup_kernel<4, 2, 2, 2><<<sp.tail, block, 0, stream>>>(sp, m, n, qs, vs, out, ldo, tiles_m,
+                                                                            tiles_n, group_m);
+        }
+        break;
     default:
         break;
     }
@@ -1957,6 +2193,6 @@ void select_rerank_i8(const SelectArgs& args, hipStream_t stream)
 
 uint32_t gram_i8_k_padding() { return kBKI8; }
 uint32_t gram_i8_max_dim() { return kI8MaxDim; }
-int gram_i8_tile_count() { return 5; }
+int gram_i8_tile_count() { return 6; }
 
 }  // namespace uhann::kern
