@@ -1589,9 +1589,11 @@ struct PpFrags {
 // K steps (128 bytes per row) reach LDS by direct buffer-to-LDS loads (no
 // staging registers, no LDS store instructions) into a ring of three stages,
 // two steps ahead. Within a stage a row's 16-byte chunk c sits in slot
-// c ^ (row % 8), so the MFMA fragment reads (32 rows of one chunk) hit every
-// bank once; a load instruction writes 8 consecutive rows (1 KB), its lanes
-// fetching the chunks in slot order.
+// c ^ ((row / 2) % 8): LDS serves 256 bytes (64 banks) per cycle, i.e. 16
+// lanes of a 16-byte read, and the 16 rows that a fragment read presents
+// together then cover the 16 bank groups once (rows of one parity share a
+// 128-byte half). A load instruction writes 8 consecutive rows (1 KB), its
+// lanes fetching the chunks in slot order.
 //
 // The two groups alternate, half a step apart, between
 //   memory phase (step s): read all 16 fragments of step s into registers,
@@ -1630,7 +1632,10 @@ gram_i8x2_pp_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, con
 
     // Loads: this wave brings query rows 128 grp + 32 lw + [0, 32) (four
     // instructions) and reference rows 64 grp + 16 lw + [0, 16) (two); lane L
-    // fetches chunk (L % 8) ^ (L / 8) of row L / 8 of the instruction's eight.
+    // fetches, for row R = R0 + L / 8 of an instruction's eight (R0 a multiple
+    // of 8), the chunk in slot L % 8: (L % 8) ^ ((R / 2) % 8), where
+    // (R / 2) % 8 = (L / 16) + 4 (R0 % 16 == 8), so it alternates between two
+    // lane patterns with the instruction's parity u % 2.
     const uint32_t ldq32 = static_cast<uint32_t>(ldq);
     const uint32_t ldv32 = static_cast<uint32_t>(ldv);
     const __amdgpu_buffer_rsrc_t rq = __builtin_amdgcn_make_buffer_rsrc(
@@ -1641,9 +1646,12 @@ gram_i8x2_pp_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, con
         kRawBufferWord3);
     const int qrow0 = 128 * grp + 32 * lw;
     const int vrow0 = 64 * grp + 16 * lw;
-    const uint32_t chunk = static_cast<uint32_t>((lane % 8) ^ (lane / 8));
-    const uint32_t qoff = static_cast<uint32_t>(qrow0 + lane / 8) * ldq32 + 16U * chunk;
-    const uint32_t voff = static_cast<uint32_t>(vrow0 + lane / 8) * ldv32 + 16U * chunk;
+    const uint32_t chunk0 = static_cast<uint32_t>((lane % 8) ^ (lane / 16));
+    const uint32_t chunk1 = chunk0 ^ 4U;
+    const uint32_t qoff0 = static_cast<uint32_t>(qrow0 + lane / 8) * ldq32 + 16U * chunk0;
+    const uint32_t qoff1 = static_cast<uint32_t>(qrow0 + lane / 8) * ldq32 + 16U * chunk1;
+    const uint32_t voff0 = static_cast<uint32_t>(vrow0 + lane / 8) * ldv32 + 16U * chunk0;
+    const uint32_t voff1 = static_cast<uint32_t>(vrow0 + lane / 8) * ldv32 + 16U * chunk1;
 
     // Fragments: lane l holds row l % 32 and bytes 16 (l / 32) .. + 15 of a
     // 32-byte sub-step (logical chunk 2 kk + l / 32 of the a plane, + 4 for
@@ -1655,7 +1663,7 @@ gram_i8x2_pp_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, con
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
-        for (int p = 0; p < 2; ++p) coff[kk][p] = 16 * ((2 * kk + 4 * p + lane / 32) ^ (lane % 8));
+        for (int p = 0; p < 2; ++p) coff[kk][p] = 16 * ((2 * kk + 4 * p + lane / 32) ^ ((lrow / 2) % 8));
 
     intx16 acc_h[2][2], acc_x[2][2];
 #pragma unroll
@@ -1672,13 +1680,13 @@ gram_i8x2_pp_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, con
         const uint32_t k = min(gb.k_begin + step * kBKI8, k_last);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rq, (lds_ptr_t)(st + (qrow0 + 8 * u) * kBKI8), 16, qoff,
-                                                     k + 8U * u * ldq32, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rq, (lds_ptr_t)(st + (qrow0 + 8 * u) * kBKI8), 16,
+                                                     u % 2 ? qoff1 : qoff0, k + 8U * u * ldq32, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rv, (lds_ptr_t)(st + (kPpBM + vrow0 + 8 * u) * kBKI8), 16, voff,
-                                                     k + 8U * u * ldv32, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rv, (lds_ptr_t)(st + (kPpBM + vrow0 + 8 * u) * kBKI8), 16,
+                                                     u % 2 ? voff1 : voff0, k + 8U * u * ldv32, 0, 0);
         }
     };
     auto read = [&](const uint8_t* st, PpFrags& f) {
