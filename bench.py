@@ -38,9 +38,10 @@ PEAK = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0, "int8": 5000.0, "hbm": 800
 
 # HBM bytes per launch of the int8 Gram kernel, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy kernel's readings
-# (profiles/exact_l2_r7028_d60660_b4096_int8_pmc_traffic.md). Keyed by
-# (references, dim, batch); other shapes report null.
-GRAM_TRAFFIC_BYTES = {}
+# (profiles/exact_l2_r7028_d60660_b4096_int8_gram.md: 7.406 GB fetched +
+# 0.179 GB written). Keyed by (references, dim, batch); other shapes report
+# null.
+GRAM_TRAFFIC_BYTES = {(7028, 60660, 4096): 7.585e9}
 
 
 def log(msg):
@@ -709,7 +710,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK["int8"], 4),
         "traffic": GRAM_TRAFFIC_BYTES.get((r, d, B)),
-        "kernel": "gram_i8x2_kernel (+ tail fix-up), dense int8 MFMA peak 5.0 POP/s; ops are int8 operations",
+        "kernel": "gram_i8x2_pp_kernel (+ tail fix-up), dense int8 MFMA peak 5.0 POP/s; ops are int8 operations",
     }
 
     cpu = None
