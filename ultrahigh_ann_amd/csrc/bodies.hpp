@@ -66,10 +66,36 @@ __device__ __forceinline__ void load_vec_stream(const float* __restrict__ p, flo
     }
 }
 
-template <int METRIC, bool WEIGHTED, bool WFMA>
-__device__ __forceinline__ float accumulate(float acc, float r, float q, float w)
+// load_vec through a raw buffer resource: vector byte offset voff (the
+// thread's column), scalar byte offset soff (the row); AUX is the cache
+// policy (2: non-temporal).
+template <int RPT, int AUX>
+__device__ __forceinline__ void load_buf(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff, float (&v)[RPT])
 {
-    const float d = r - q;
+    if constexpr (RPT == 1) {
+        v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, AUX));
+    } else if constexpr (RPT == 2) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, AUX);
+        v[0] = __uint_as_float(x[0]);
+        v[1] = __uint_as_float(x[1]);
+    } else {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, AUX);
+        v[0] = __uint_as_float(x[0]);
+        v[1] = __uint_as_float(x[1]);
+        v[2] = __uint_as_float(x[2]);
+        v[3] = __uint_as_float(x[3]);
+    }
+}
+
+template <bool V>
+struct BoolTag {
+    static constexpr bool value = V;
+};
+
+// One chain step from the difference d = r - q (binary32).
+template <int METRIC, bool WEIGHTED, bool WFMA>
+__device__ __forceinline__ float accumulate_diff(float acc, float d, float w)
+{
     if constexpr (METRIC == 0) {
         return __builtin_fmaf(d, d, acc);
     } else {
@@ -82,6 +108,33 @@ __device__ __forceinline__ float accumulate(float acc, float r, float q, float w
             const float prod = w * ad;
             return acc + prod;
         }
+    }
+}
+
+template <int METRIC, bool WEIGHTED, bool WFMA>
+__device__ __forceinline__ float accumulate(float acc, float r, float q, float w)
+{
+    return accumulate_diff<METRIC, WEIGHTED, WFMA>(acc, r - q, w);
+}
+
+// An empty asm statement that reads and writes v[0 .. N): every value is
+// formed before it and used after it, so the instructions that produce a
+// group of values issue together and their consumers follow together (a
+// dependent pair issued back to back waits for the first result).
+template <int N>
+__device__ __forceinline__ void issue_fence(float* v)
+{
+    if constexpr (N >= 8) {
+        asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]), "+v"(v[7]));
+        issue_fence<N - 8>(v + 8);
+    } else if constexpr (N >= 4) {
+        asm volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]));
+        issue_fence<N - 4>(v + 4);
+    } else if constexpr (N >= 2) {
+        asm volatile("" : "+v"(v[0]), "+v"(v[1]));
+        issue_fence<N - 2>(v + 2);
+    } else if constexpr (N == 1) {
+        asm volatile("" : "+v"(v[0]));
     }
 }
 
@@ -112,23 +165,57 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
 #pragma unroll
             for (int k = 0; k < RPT; ++k) acc[i][j][k] = 0.0F;
 
+    // Buffer form of the loads (a.buffers): one resource at the tile's first
+    // reference row and one at its first query row; reference j and query i
+    // are scalar byte offsets from them (clamped to the last row like the
+    // pointers above) and the column is the thread's one vector offset, so
+    // no 64-bit address is formed per load.
+    const __amdgpu_buffer_rsrc_t rres = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(a.refs + static_cast<uint64_t>(r0) * a.ld_ref), 0, -1, kBufferWord3);
+    const __amdgpu_buffer_rsrc_t qres = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(a.queries + static_cast<uint64_t>(min(q0, a.nq - 1U)) * a.ld_q), 0, -1, kBufferWord3);
+    uint32_t roff[TR];
+    uint32_t qoff[TQ];
+#pragma unroll
+    for (int j = 0; j < TR; ++j) {
+        roff[j] = min(static_cast<uint32_t>(j), a.rows - 1U - r0) * static_cast<uint32_t>(a.ld_ref) * 4U;
+    }
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) {
+        qoff[i] = q0 < a.nq ? min(static_cast<uint32_t>(i), a.nq - 1U - q0) * static_cast<uint32_t>(a.ld_q) * 4U : 0U;
+    }
+
     // One 256-column step at column `col` (this thread's first chain). In the
     // tail step a vector may straddle dim (rows are padded to a multiple of 4
     // floats): components at or past dim are skipped, so pad values are
     // never added.
-    auto step = [&](uint32_t col, bool masked, bool stream) {
+    auto step = [&](uint32_t col, bool masked, bool stream, auto buf_tag) {
+        constexpr bool BUF = decltype(buf_tag)::value;
         float rv[TR][RPT];
         float qv[TQ][RPT];
         float wv[RPT];
-        if (stream) {
+        if constexpr (BUF) {
+            const uint32_t voff = col * 4U;
+            if (stream) {
 #pragma unroll
-            for (int j = 0; j < TR; ++j) load_vec_stream<RPT>(rrow[j] + col, rv[j]);
+                for (int j = 0; j < TR; ++j) load_buf<RPT, 2>(rres, voff, roff[j], rv[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < TR; ++j) load_buf<RPT, 0>(rres, voff, roff[j], rv[j]);
+            }
+#pragma unroll
+            for (int i = 0; i < TQ; ++i) load_buf<RPT, 0>(qres, voff, qoff[i], qv[i]);
         } else {
+            if (stream) {
 #pragma unroll
-            for (int j = 0; j < TR; ++j) load_vec<RPT>(rrow[j] + col, rv[j]);
+                for (int j = 0; j < TR; ++j) load_vec_stream<RPT>(rrow[j] + col, rv[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < TR; ++j) load_vec<RPT>(rrow[j] + col, rv[j]);
+            }
+#pragma unroll
+            for (int i = 0; i < TQ; ++i) load_vec<RPT>(qrow[i] + col, qv[i]);
         }
-#pragma unroll
-        for (int i = 0; i < TQ; ++i) load_vec<RPT>(qrow[i] + col, qv[i]);
         if constexpr (WEIGHTED) {
             load_vec<RPT>(a.weights + col, wv);
         } else {
@@ -139,11 +226,16 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
         for (int k = 0; k < RPT; ++k) {
             if (masked && col + static_cast<uint32_t>(k) >= a.dim) continue;
 #pragma unroll
-            for (int i = 0; i < TQ; ++i)
+            for (int i = 0; i < TQ; ++i) {
+                // The TR differences first, then their TR chain steps.
+                float dd[TR];
+#pragma unroll
+                for (int j = 0; j < TR; ++j) dd[j] = rv[j][k] - qv[i][k];
+                issue_fence<TR>(dd);
 #pragma unroll
                 for (int j = 0; j < TR; ++j)
-                    acc[i][j][k] = accumulate<METRIC, WEIGHTED, WFMA>(acc[i][j][k], rv[j][k],
-                                                                     qv[i][k], wv[k]);
+                    acc[i][j][k] = accumulate_diff<METRIC, WEIGHTED, WFMA>(acc[i][j][k], dd[j], wv[k]);
+            }
         }
     };
 
@@ -218,16 +310,24 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
         }
     }
     if (!rows_done) {
-        if (a.stream_refs) {
-            // Single-pass streams keep more 256-column steps of loads in flight.
-            constexpr int kStreamUnroll = RPT == 4 && TQ * TR <= 8 ? 4 : 2;
+        // Single-pass streams keep more 256-column steps of loads in flight.
+        constexpr int kStreamUnroll = RPT == 4 && TQ * TR <= 8 ? 4 : 2;
+        if (a.buffers) {
+            if (a.stream_refs) {
 #pragma unroll kStreamUnroll
-            for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, true);
+                for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, true, BoolTag<true>{});
+            } else {
+#pragma unroll 2
+                for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, false, BoolTag<true>{});
+            }
+        } else if (a.stream_refs) {
+#pragma unroll kStreamUnroll
+            for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, true, BoolTag<false>{});
         } else {
 #pragma unroll 2
-            for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, false);
+            for (uint32_t c0 = 0; c0 < full_end; c0 += kChains) step(c0 + col_t, false, false, BoolTag<false>{});
         }
-        if (full_end + col_t < a.dim) step(full_end + col_t, true, false);
+        if (full_end + col_t < a.dim) step(full_end + col_t, true, false, BoolTag<false>{});
 #pragma unroll
         for (int i = 0; i < TQ; ++i) {
 #pragma unroll

@@ -271,12 +271,13 @@ kern::ScanTile choose_tile(uint64_t nq, bool aligned4)
     } else if (nq <= 16) {
         t = kern::ScanTile{1, 8, 16, 2};
     } else {
-        t = kern::ScanTile{1, 12, 12, 8};
+        t = kern::ScanTile{1, 16, 8, 4};
     }
     return t;
 }
 
 bool stream_refs_enabled();
+bool scan_buffers_enabled();
 
 void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t count,
                  unsigned long long* keys, uint64_t* out_idx, float* out_dist,
@@ -344,6 +345,10 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     }
     // One query tile reads every reference value exactly once: stream them.
     a.stream_refs = a.nq <= static_cast<uint32_t>(tile.tq) && stream_refs_enabled();
+    // Buffer-resource loads when a tile's rows lie within 2^32 bytes of its
+    // first row (references and queries).
+    a.buffers = scan_buffers_enabled() && static_cast<uint64_t>(tile.tr) * scan.ld * 4U < (1ULL << 32) &&
+                static_cast<uint64_t>(tile.tq) * ld_q * 4U < (1ULL << 32);
     if (a.dim == 0) {
         // No coordinates: every distance is zero and the reference returns row 0.
         a.queries = scan.refs.data();
@@ -371,6 +376,14 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
 bool stream_refs_enabled()
 {
     const char* env = std::getenv("UHANN_STREAM_REFS");
+    return env == nullptr || env[0] != '0';
+}
+
+// Buffer-resource loads in the tiled direct scan; UHANN_SCAN_BUFFERS=0 uses
+// 64-bit global addresses instead (same arithmetic, for measurement).
+bool scan_buffers_enabled()
+{
+    const char* env = std::getenv("UHANN_SCAN_BUFFERS");
     return env == nullptr || env[0] != '0';
 }
 

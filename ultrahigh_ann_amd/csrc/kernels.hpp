@@ -44,6 +44,10 @@ struct ScanArgs {
     // load references with the non-temporal cache policy so the stream does
     // not evict the queries from L2.
     bool stream_refs = false;
+    // Tile loads through buffer resources (one at the tile's first reference
+    // row, one at its first query row, rows as scalar byte offsets): set by
+    // the launcher when every such offset stays below 2^32.
+    bool buffers = false;
 };
 
 // Register tile of the direct scan: a block of 256/rpt threads, each thread
