@@ -306,9 +306,11 @@ def serving(uh, torch, refs_np, queries, args, n_indexes=64):
            "pairs_tq_autotuned": tq, "queries_on_device": True}
     for label, ids in (("fp32", ids32), ("mixed", ids_mixed)):
         burst = run(ids, 3000, None)
-        rate = 0.7 * burst["requests_per_s"]
-        n = int(min(20000, max(500, rate * 2.0)))
-        out[label] = {"burst": burst, "poisson_70pct": dict(run(ids, n, rate), offered_requests_per_s=round(rate, 1))}
+        out[label] = {"burst": burst}
+        if not args.no_poisson:
+            rate = 0.7 * burst["requests_per_s"]
+            n = int(min(20000, max(500, rate * 2.0)))
+            out[label]["poisson_70pct"] = dict(run(ids, n, rate), offered_requests_per_s=round(rate, 1))
         log(f"serving {label}: {out[label]}")
     srv.close()
     return out
@@ -591,6 +593,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-suite", action="store_true", help="skip the secondary workloads")
     ap.add_argument("--suite", default="", help="comma-separated secondary workloads to run (default: all)")
+    ap.add_argument("--suite-only", action="store_true",
+                    help="run only the secondary workloads (for profiling them); prints {\"workloads\": ...}")
+    ap.add_argument("--no-poisson", action="store_true",
+                    help="serving: the burst only, without the Poisson latency runs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -623,6 +629,15 @@ def main():
     log(f"generating synthetic data r={r} d={d} B={B} seed={args.seed}")
     refs, queries = synthetic.log_expression(r, B, d, seed=args.seed, device="cuda")
     refs_np = refs.cpu().numpy()
+    if args.suite_only:
+        # the exact L2 argmins the quality figures compare against (binary64, GPU table)
+        ex = uh.HipExactL2Index(refs_np)
+        table, best = fp64_truth(torch, ex, queries, B)
+        ex_idx = best.cpu().numpy().astype(np.uint64)
+        del ex, table
+        torch.cuda.empty_cache()
+        print(json.dumps({"workloads": suite(uh, torch, refs, queries, refs_np, ex_idx, args)}), flush=True)
+        return
     log("building index (upload references, int8 / bf16 / fp8 copies)")
     index = uh.HipExactL2Index(refs_np)
     for strat in (uh.STRATEGY_GRAM_INT8, uh.STRATEGY_GRAM_BF16, uh.STRATEGY_GRAM_FP8):
