@@ -187,6 +187,14 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
  * cuda_dense_l2_scan_index.cu:449-462). */
 int uhann_scan_enable_mixed(uhann_scan* scan, int strategy);
 
+/* Rows the GRAM_FP8 strategy re-ranks exactly per query: the `rows` smallest
+ * FP8 estimates (1 .. 16; 16 when never set). Fewer rows read fewer
+ * reference rows in the re-rank; the shortlist is not certified either way
+ * (uhann_workspace_shortlist_stats counts the queries it provably holds).
+ * Call before sharing the handle between threads. No counterpart in the
+ * reference (its unrefined SGEMM keeps no shortlist). */
+int uhann_scan_set_fp8_shortlist(uhann_scan* scan, uint32_t rows);
+
 /* Counters of the last GRAM_BF16 batch on this workspace: total candidate
  * set size, queries decided in binary64, queries escalated to the direct
  * scan. Synchronises the workspace stream. */

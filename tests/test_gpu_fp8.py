@@ -114,7 +114,7 @@ def test_fp8_estimates_exact_on_small_integers(uh, d):
     np.testing.assert_array_equal(out.cpu().numpy().astype(np.float64), want)
 
 
-def _check_results(uh, oracle_mod, refs, q, index, ws, idx, dist):
+def _check_results(uh, oracle_mod, refs, q, index, ws, idx, dist, k=16):
     import torch
 
     b, r = q.shape[0], refs.shape[0]
@@ -127,8 +127,8 @@ def _check_results(uh, oracle_mod, refs, q, index, ws, idx, dist):
     index.fp8_estimates(torch.from_numpy(q).cuda(), b, ws, out)
     torch.cuda.synchronize()
     est = out.cpu().numpy()
-    # shortlist = 16 smallest estimates (ties to the lower row)
-    order = np.lexsort((np.tile(np.arange(r), (b, 1)), est), axis=1)[:, : min(16, r)]
+    # shortlist = k smallest estimates (ties to the lower row)
+    order = np.lexsort((np.tile(np.arange(r), (b, 1)), est), axis=1)[:, : min(k, r)]
     in_short = np.array([chosen[i] in order[i] for i in range(b)])
     assert in_short.all()
     exp_idx, _ = oracle_mod.fp32_argmin(k2)
@@ -152,6 +152,25 @@ def test_fp8_strategy_random(uh, oracle_mod, r, d, b):
     st = ws.mixed_stats()
     assert st["candidates"] == b * min(16, r) and st["escalated"] == 0
     assert 0 <= ws.shortlist_stats()["certified"] <= b
+
+
+@pytest.mark.parametrize("k", [1, 4, 8])
+def test_fp8_shortlist_length(uh, oracle_mod, k):
+    """A shorter shortlist re-ranks exactly the k smallest estimates: the
+    answer is a shortlisted row's binary32 distance, and the direct answer
+    when that row is shortlisted; lengths outside 1..16 are rejected."""
+    r, d, b = 300, 1000, 40
+    refs, q = _case(7 * r + d, r, d, b)
+    index = uh.HipExactL2Index(refs)
+    index.enable_mixed(uh.STRATEGY_GRAM_FP8)
+    index.set_fp8_shortlist(k)
+    ws = index.make_query_workspace(b)
+    idx, dist = index.query_batch(q, b, ws, uh.STRATEGY_GRAM_FP8)
+    _check_results(uh, oracle_mod, refs, q, index, ws, idx, dist, k)
+    assert ws.mixed_stats()["candidates"] == b * k
+    for bad in (0, 17):
+        with pytest.raises(uh.UhannInvalidArgument):
+            index.set_fp8_shortlist(bad)
 
 
 def test_fp8_strategy_log_expression(uh, oracle_mod):

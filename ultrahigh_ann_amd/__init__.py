@@ -260,6 +260,10 @@ class _DenseScan:
         """Prepare the BF16 reference copy for the mixed-precision strategy."""
         check(_native.lib().uhann_scan_enable_mixed(self._h, strategy))
 
+    def set_fp8_shortlist(self, rows: int):
+        """Rows the GRAM_FP8 strategy re-ranks per query (1 .. 16, default 16)."""
+        check(_native.lib().uhann_scan_set_fp8_shortlist(self._h, rows))
+
     def make_query_workspace(self, maximum_batch_size: int) -> QueryWorkspace:
         h = c_void_p()
         check(_native.lib().uhann_scan_workspace_create(self._h, maximum_batch_size, ctypes.byref(h)))

@@ -121,6 +121,7 @@ def _declare(lib):
              c_int, c_int, c_void_p],
         ),
         "uhann_scan_enable_mixed": (c_int, [c_void_p, c_int]),
+        "uhann_scan_set_fp8_shortlist": (c_int, [c_void_p, ctypes.c_uint32]),
         "uhann_workspace_mixed_stats": (c_int, [c_void_p, u64p, u64p, u64p]),
         "uhann_workspace_set_timing": (c_int, [c_void_p, c_int]),
         "uhann_workspace_stage_times": (c_int, [c_void_p, f32p, c_int]),

@@ -633,6 +633,7 @@ void fp8_launch(const Scan& scan, Workspace& ws, const float* queries, uint64_t 
     a.out_idx = out_idx;
     a.out_dist = out_dist;
     a.stats = reinterpret_cast<kern::MixedStats*>(ws.counters.data() + 4);
+    a.shortlist = scan.fp8_shortlist;
     kern::select_topk_rerank(a, stream);
     ws.mark(3, stream);
     ws.mark(4, stream);
@@ -1099,6 +1100,17 @@ int uhann_scan_enable_mixed(uhann_scan* scan, int strategy)
     return guarded([&] {
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
         scan->enable_mixed(strategy);
+    });
+}
+
+int uhann_scan_set_fp8_shortlist(uhann_scan* scan, uint32_t rows)
+{
+    return guarded([&] {
+        if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
+        if (rows < 1 || rows > static_cast<uint32_t>(kern::kShortlist)) {
+            throw std::invalid_argument("the FP8 shortlist holds 1 to 16 rows");
+        }
+        scan->fp8_shortlist = rows;
     });
 }
 

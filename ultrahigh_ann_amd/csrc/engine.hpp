@@ -67,6 +67,7 @@ struct Scan {
     std::unique_ptr<GramRefs> gram;  // set by enable_mixed(GRAM_BF16)
     std::unique_ptr<Fp8Refs> fp8;    // set by enable_mixed(GRAM_FP8)
     std::unique_ptr<I8Refs> i8;      // set by enable_mixed(GRAM_INT8)
+    uint32_t fp8_shortlist = kern::kShortlist;  // GRAM_FP8: rows re-ranked per query
 
     // Autotuned direct-scan kernels per batch bucket b (batches of
     // 2^(b-1)+1 .. 2^b queries); empty entries fall back to the heuristic.

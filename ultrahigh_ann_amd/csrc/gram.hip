@@ -1085,10 +1085,10 @@ gram_fp8_kernel(const uint8_t* __restrict__ Q, uint64_t ldq, uint32_t M, const u
     gram_epilogue<2, 2, true>(acc, m0 + wm, n0 + wn, lane, M, N, qs, vs, out, ldo);
 }
 
-// Top-kShortlist selection per query (64-bit keys (ordered estimate << 32 |
-// row): per-thread sorted lists, then an 8-pass radix select of the
-// kShortlist-th smallest key over their union), exact re-rank, and the
-// certification pass.
+// Top-k selection per query, k = a.shortlist <= kShortlist (64-bit keys
+// (ordered estimate << 32 | row): per-thread sorted lists of kShortlist,
+// then an 8-pass radix select of the k-th smallest key over their union),
+// exact re-rank, and the certification pass.
 __global__ void __launch_bounds__(kSelThreads)
 select_topk_rerank_kernel(SelectArgs a)
 {
@@ -1127,7 +1127,7 @@ select_topk_rerank_kernel(SelectArgs a)
             }
         }
     }
-    const uint32_t need0 = min(static_cast<uint32_t>(kShortlist), a.rows);
+    const uint32_t need0 = min(min(a.shortlist, static_cast<uint32_t>(kShortlist)), a.rows);
     if (t == 0) {
         s_prefix = 0ULL;
         s_need = need0;

@@ -80,6 +80,7 @@ struct SelectArgs {
     unsigned int* escalate_count = nullptr;
     unsigned int* escalate_list = nullptr;  // capacity nq
     MixedStats* stats = nullptr;
+    uint32_t shortlist = kShortlist;  // FP8 strategy: rows re-ranked per query (1 .. kShortlist)
 };
 
 void to_bf16_rows(const float* src, uint64_t ld_src, uint32_t rows, uint32_t dim, __bf16* dst,
