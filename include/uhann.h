@@ -168,9 +168,11 @@ int uhann_scan_query_device_noalloc(const uhann_scan* scan, const float* device_
  * chains per thread (1, 2 or 4), tq x tr (query, reference) pairs per block,
  * group_q query tiles per reference sweep; rpt = 0 selects the
  * pair-per-thread scan for rows of at most 4096 coordinates (and fewer than
- * 2^25 reference rows) with tq (1, 2, 4) queries per thread (the default for
- * such rows is tq = 2), and rpt = 0, tq = 0 the thread-per-query scan for rows
- * of 1 to 128 coordinates. Returns INVALID_ARGUMENT for a tile that is not
+ * 2^25 reference rows) with tq (1, 2, 4) queries per thread, or with tr = 8
+ * its chain-class form with tq (4, 8) queries per thread (the default for
+ * such rows is the chain-class form with tq = 4 up to 320 coordinates, tq = 2
+ * above), and rpt = 0, tq = 0 the thread-per-query scan for rows of 1 to 128
+ * coordinates. Returns INVALID_ARGUMENT for a tile that is not
  * compiled in or does not fit the rows. No counterpart in the reference:
  * used by the autotuner and by the tests that cover every tile. */
 int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,

@@ -15,7 +15,7 @@ TILES = [(1, 8, 8, 8), (1, 12, 12, 8), (1, 16, 8, 4), (1, 8, 16, 2), (1, 4, 16, 
          (1, 1, 16, 1), (4, 1, 8, 1), (4, 2, 8, 1), (4, 4, 8, 1), (4, 8, 4, 2),
          (1, 1, 4, 1), (1, 2, 8, 1), (1, 4, 8, 1), (1, 16, 4, 4), (4, 1, 4, 1), (4, 2, 4, 1), (4, 4, 4, 1),
          (4, 1, 2, 1), (4, 1, 1, 1), (4, 2, 2, 1), (4, 4, 2, 1),
-         (0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0), (0, 0, 0, 0)]
+         (0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0), (0, 4, 8, 0), (0, 8, 8, 0), (0, 0, 0, 0)]
 
 
 def test_exact_l2_reference_fixture(uh):
@@ -144,17 +144,19 @@ def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
     np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
 
 
-@pytest.mark.parametrize("tq", [1, 2, 4])
+@pytest.mark.parametrize("tile", [(0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0), (0, 4, 8, 0), (0, 8, 8, 0)])
 @pytest.mark.parametrize("metric", ["l2", "l1", "l1w"])
-@pytest.mark.parametrize("d", [222, 300, 32])
-def test_pair_tiles_many_reference_blocks(uh, oracle_mod, tq, metric, d):
+@pytest.mark.parametrize("d", [222, 300, 32, 1000, 4096])
+def test_pair_tiles_many_reference_blocks(uh, oracle_mod, tile, metric, d):
     # Pair-per-thread scan over several 256-row reference blocks (the last one
     # partial), whole 32-column chunks and tails, empty groups (d = 32, 222),
-    # and the L1 weights, for every queries-per-thread width.
+    # several terms per chain (d = 1000, 4096: the widest rows), and the L1
+    # weights, for every queries-per-thread width of both bodies (whole
+    # 32-chain groups; chain classes, tr = 8).
     import torch
 
     r, b = 600, 9
-    refs, q = _data(11 * d + tq, r, d, b)
+    refs, q = _data(11 * d + tile[1], r, d, b)
     if metric == "l2":
         index = uh.HipDenseL2ScanIndex(refs)
         expect = oracle_mod.k2_distances(refs, q)
@@ -170,7 +172,7 @@ def test_pair_tiles_many_reference_blocks(uh, oracle_mod, tq, metric, d):
     dq = torch.from_numpy(q).cuda()
     oi = torch.zeros(b, dtype=torch.int64, device="cuda")
     od = torch.zeros(b, dtype=torch.float32, device="cuda")
-    index.query_device_tile(dq, b, ws, oi, od, (0, tq, 0, 0), stream=torch.cuda.current_stream())
+    index.query_device_tile(dq, b, ws, oi, od, tile, stream=torch.cuda.current_stream())
     torch.cuda.synchronize()
     np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
     np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))

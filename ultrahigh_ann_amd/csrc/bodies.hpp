@@ -522,6 +522,184 @@ __device__ __forceinline__ void pair_group_terms(float (&v)[TQ][32], const float
     }
 }
 
+// Query values of column c for TQ queries from the column-major LDS copy
+// (all lanes read the same address: a broadcast).
+template <int TQ>
+__device__ __forceinline__ void lds_query_column(const float* qs, uint32_t c, float (&qv)[TQ])
+{
+    const float* p = qs + static_cast<uint64_t>(c) * TQ;
+    if constexpr (TQ % 4 == 0) {
+#pragma unroll
+        for (int k = 0; k < TQ / 4; ++k) {
+            const float4 x = *reinterpret_cast<const float4*>(p + 4 * k);
+            qv[4 * k] = x.x;
+            qv[4 * k + 1] = x.y;
+            qv[4 * k + 2] = x.z;
+            qv[4 * k + 3] = x.w;
+        }
+    } else if constexpr (TQ == 2) {
+        const float2 x = *reinterpret_cast<const float2*>(p);
+        qv[0] = x.x;
+        qv[1] = x.y;
+    } else {
+        qv[0] = p[0];
+    }
+}
+
+// One chain class of a group in the pair-per-thread scan: chains 32w + m + 4s
+// (s = 0..7) of this thread's reference row, every term of each chain in
+// column order, then their part of the group's 32-lane tree (offsets 16, 8,
+// 4 between chains are s offsets 4, 2, 1). cls[q] receives the class value
+// (the group sum is (d0 + d2) + (d1 + d3) over the classes m).
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__device__ __forceinline__ void pair_class(float (&cls)[TQ], const float* __restrict__ refs_t, uint64_t ld_t,
+                                           uint32_t roff, const float* qs, const float* ws, uint32_t first,
+                                           uint32_t dim)
+{
+    float v[TQ][8];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) v[q][s] = 0.0F;
+    const uint32_t col_bytes = static_cast<uint32_t>(ld_t) * 4U;
+    for (uint32_t base = first; base < dim; base += kChains) {
+        // Columns base + 4s: the resource sits at column base, column
+        // base + 4s is 4s rows of refs_t past it (rows < 2^25 keep every
+        // offset below 2^32); a column at or past dim is read clamped and
+        // not added.
+        const __amdgpu_buffer_rsrc_t cols = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(refs_t + static_cast<uint64_t>(base) * ld_t), 0, -1, kBufferWord3);
+        float rv[8];
+        const bool full = base + 28U < dim;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t c = full ? base + 4U * s : min(base + 4U * s, dim - 1U);
+            rv[s] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(cols, roff, (c - base) * col_bytes, 0));
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const uint32_t c = base + 4U * s;
+            if (full || c < dim) {
+                float qv[TQ];
+                lds_query_column<TQ>(qs, c, qv);
+                const float wv = WEIGHTED ? ws[c] : 0.0F;
+                float dd[TQ];
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) dd[q] = rv[s] - qv[q];
+                issue_fence<TQ>(dd);
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) v[q][s] = accumulate_diff<METRIC, WEIGHTED, WFMA>(v[q][s], dd[q], wv);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+#pragma unroll
+        for (int off = 4; off >= 1; off /= 2) {
+#pragma unroll
+            for (int s = 0; s < off; ++s) v[q][s] = v[q][s] + v[q][s + off];
+        }
+        cls[q] = v[q][0];
+    }
+}
+
+// Pair-per-thread scan by chain classes (the default for short rows): each
+// thread owns one reference row and TQ <= 8 queries. A group of 32 chains is
+// taken a chain class at a time (chains 32w + m + 4s, m in the tree order
+// 0, 2, 1, 3), so 8 x TQ partial sums stay live instead of 32 x TQ and a
+// loaded reference value serves up to 8 queries; each chain still adds its
+// terms in column order and the additions of the tree are the reference's
+// (W_w = (d0 + d2) + (d1 + d3) with d_m the class sums, then
+// ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7))), so the bits are the reference's.
+// Queries (and L1 weights) sit in LDS column-major (the TQ values of a
+// column together, read as broadcast vectors).
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__device__ __forceinline__ void scan_pairs_cls(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
+                                               const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
+                                               const float* __restrict__ weights, uint32_t dim,
+                                               unsigned long long* __restrict__ keys, uint32_t q0, uint32_t rblock,
+                                               float* smem)
+{
+    refs_t = reinterpret_cast<const float*>(uniform_u64(reinterpret_cast<uint64_t>(refs_t)));
+    ld_t = uniform_u64(ld_t);
+    rows = __builtin_amdgcn_readfirstlane(rows);
+    dim = __builtin_amdgcn_readfirstlane(dim);
+    const uint32_t ldqs = (dim + 3U) & ~3U;
+    float* qs = smem;               // ldqs x TQ, column-major
+    float* ws = smem + TQ * ldqs;   // ldqs (weighted L1)
+    const int t = threadIdx.x;
+    for (uint32_t e = t; e < TQ * ldqs; e += 256) {
+        const uint32_t qi = e / ldqs;
+        const uint32_t c = e % ldqs;
+        qs[c * TQ + qi] = c < dim ? queries[static_cast<uint64_t>(min(q0 + qi, nq - 1U)) * ld_q + c] : 0.0F;
+    }
+    if constexpr (WEIGHTED) {
+        for (uint32_t c = t; c < ldqs; c += 256) ws[c] = c < dim ? weights[c] : 0.0F;
+    }
+    __syncthreads();
+    const uint32_t r = rblock * 256 + t;
+    const uint32_t roff = min(r, rows - 1U) * 4U;
+    // Groups in bit-reversed order 0,4 | 2,6 | 1,5 | 3,7: pass p produces
+    // (W0+W4), (W2+W6), (W1+W5), (W3+W7) in turn, and the distance is
+    // (P0 + P1) + (P2 + P3).
+    float P[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[q][k] = 0.0F;
+#pragma unroll 1
+    for (int pass = 0; pass < 4; ++pass) {
+        float pair_sum[TQ];
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            const int g = 2 * pass + h;
+            const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
+            const uint32_t gbase = static_cast<uint32_t>(32 * w);
+            float gsum[TQ];
+            if (gbase >= dim) {
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) gsum[q] = 0.0F;
+            } else {
+                float e0[TQ], d[TQ];
+                pair_class<METRIC, WEIGHTED, WFMA, TQ>(e0, refs_t, ld_t, roff, qs, ws, gbase + 0U, dim);
+                pair_class<METRIC, WEIGHTED, WFMA, TQ>(d, refs_t, ld_t, roff, qs, ws, gbase + 2U, dim);
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) e0[q] = e0[q] + d[q];
+                float e1[TQ];
+                pair_class<METRIC, WEIGHTED, WFMA, TQ>(e1, refs_t, ld_t, roff, qs, ws, gbase + 1U, dim);
+                pair_class<METRIC, WEIGHTED, WFMA, TQ>(d, refs_t, ld_t, roff, qs, ws, gbase + 3U, dim);
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) gsum[q] = e0[q] + (e1[q] + d[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) pair_sum[q] = h == 0 ? gsum[q] : pair_sum[q] + gsum[q];
+        }
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) P[q][k] = pass == k ? pair_sum[q] : P[q][k];
+    }
+    __shared__ unsigned long long best[TQ][4];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) {
+        const float dist = (P[q][0] + P[q][1]) + (P[q][2] + P[q][3]);
+        unsigned long long key = r < rows
+                                     ? (static_cast<unsigned long long>(__float_as_uint(dist)) << 32) | r
+                                     : ~0ULL;
+        for (int off = 32; off >= 1; off /= 2) {
+            const unsigned long long o = __shfl_down(key, off);
+            key = o < key ? o : key;
+        }
+        if (t % 64 == 0) best[q][t / 64] = key;
+    }
+    __syncthreads();
+    if (t < TQ && q0 + t < nq) {
+        unsigned long long b = best[t][0];
+        for (int w = 1; w < 4; ++w) b = best[t][w] < b ? best[t][w] : b;
+        atomicMin(keys + q0 + t, b);
+    }
+}
+
 // Pair-per-thread form for short rows (flat |I|, hierarchical p): the
 // 256-chain tree costs 255 adds per pair whatever the row length, and in the
 // tiled kernel above it is a cross-lane tree, which dominates when a chain

@@ -203,6 +203,7 @@ void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_
         }
         if (scan.refs_t.bytes() != 0 && scan.dim != 0 && scan.rows < kern::kPairsMaxRows) {
             for (int tq : {1, 2, 4}) cands.push_back(kern::ScanTile{0, tq, 0, 0});
+            for (int tq : {4, 8}) cands.push_back(kern::ScanTile{0, tq, 8, 0});
         }
         if (scan.dim != 0 && scan.dim <= kern::scan_short_max_dim()) cands.push_back(kern::ScanTile{0, 0, 0, 0});
         for (uint64_t nq = 1;; nq *= 2) {
@@ -314,7 +315,9 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     } else if (forced_tile == nullptr && short_ok && short_blocks >= device_cus(scan.device)) {
         tile = kern::ScanTile{0, 0, 0, 0};
     } else if (forced_tile == nullptr && pairs_ok && a.dim <= pairs_max_dim() && count >= 4) {
-        tile = kern::ScanTile{0, 2, 0, 0};
+        // chain classes, 4 queries per thread, for rows up to 320 coordinates
+        // (faster there); whole groups, 2 queries per thread, above
+        tile = a.dim <= 320 ? kern::ScanTile{0, 4, 8, 0} : kern::ScanTile{0, 2, 0, 0};
     }
     if (tile.rpt == 0) {
         const kern::Metric metric = scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1;
@@ -327,7 +330,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
                              a.weights, a.dim, keys, stream);
         } else {
             kern::scan_pairs(metric, /*weight_fma=*/true, scan.refs_t.data(), scan.rows, a.rows, queries, ld_q, a.nq,
-                             a.weights, a.dim, keys, tile.tq, stream);
+                             a.weights, a.dim, keys, tile.tq, tile.tr == 8, stream);
         }
         if (timed != nullptr) timed->mark(1, stream);
         kern::finalize_keys(keys, a.nq, out_idx, out_dist, stream);
@@ -929,7 +932,8 @@ int uhann_scan_query_device_tile(const uhann_scan* scan, uhann_workspace* ws,
         if (scan == nullptr) throw std::invalid_argument("scan handle must not be null");
         checked_ws(scan, ws, count);
         const kern::ScanTile tile{rpt, tq, tr, group_q};
-        const bool pair_tile = rpt == 0 && (tq == 0 || tq == 1 || tq == 2 || tq == 4);
+        const bool pair_tile = rpt == 0 && ((tr == 0 && (tq == 0 || tq == 1 || tq == 2 || tq == 4)) ||
+                                            (tr == 8 && (tq == 4 || tq == 8)));
         if (!pair_tile && !kern::scan_tile_supported(tile)) throw std::invalid_argument("unsupported scan tile");
         if (count == 0) return;
         if (scan->dim != 0 && device_queries == nullptr) {

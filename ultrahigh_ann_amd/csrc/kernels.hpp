@@ -92,14 +92,15 @@ void scan_short(Metric metric, bool weight_fma, const float* refs, uint64_t ld_r
                 unsigned long long* keys, hipStream_t stream);
 
 // Pair-per-thread direct scan for dim <= scan_pairs_max_dim(): refs_t is the
-// transposed (dim x rows, row stride ld_t) reference matrix; tq (1, 2, 4)
-// queries per block, rows < kPairsMaxRows. Same keys contract as
+// transposed (dim x rows, row stride ld_t) reference matrix; tq queries per
+// block, rows < kPairsMaxRows. classes: the chain-class body (tq 4 or 8)
+// instead of whole 32-chain groups (tq 1, 2 or 4). Same keys contract as
 // scan_direct, same bits.
 constexpr uint32_t kPairsMaxRows = 1U << 25;
 uint32_t scan_pairs_max_dim();
 void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld_t, uint32_t rows,
                 const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,
-                unsigned long long* keys, int tq, hipStream_t stream);
+                unsigned long long* keys, int tq, bool classes, hipStream_t stream);
 
 // Gather of sampled coordinates: out[q][j] = src[q][cols[j]] (* scales[j]
 // when scales is non-null), for q < nq, j < k. Sets *invalid to 1 when a

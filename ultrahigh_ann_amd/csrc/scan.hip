@@ -82,6 +82,19 @@ scan_pairs_kernel(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows
                                                  blockIdx.y * TQ, blockIdx.x, smem);
 }
 
+// Chain-class form of the pair-per-thread scan (body::scan_pairs_cls): grid
+// as scan_pairs_kernel.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__global__ void __launch_bounds__(256)
+scan_pairs_cls_kernel(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
+                      const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
+                      const float* __restrict__ weights, uint32_t dim, unsigned long long* __restrict__ keys)
+{
+    extern __shared__ float smem[];
+    body::scan_pairs_cls<METRIC, WEIGHTED, WFMA, TQ>(refs_t, ld_t, rows, queries, ld_q, nq, weights, dim, keys,
+                                                     blockIdx.y * TQ, blockIdx.x, smem);
+}
+
 // Device-side escalation of the mixed-precision strategy (no host round
 // trip): the queries its selection listed (list[0 .. *count)) are copied into
 // a contiguous block (their keys reset on the way), scanned exactly by a
@@ -316,7 +329,7 @@ uint32_t scan_pairs_max_dim() { return 4096; }
 
 void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld_t, uint32_t rows,
                 const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,
-                unsigned long long* keys, int tq, hipStream_t stream)
+                unsigned long long* keys, int tq, bool classes, hipStream_t stream)
 {
     if (nq == 0 || rows == 0) return;
     if (dim > scan_pairs_max_dim()) throw std::invalid_argument("pair scan supports dim <= 4096");
@@ -324,7 +337,11 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     if (rows >= kPairsMaxRows) throw std::length_error("pair scan supports fewer than 2^25 reference rows");
     const size_t ldqs = (static_cast<size_t>(dim) + 3) & ~static_cast<size_t>(3);
     const size_t lds = (static_cast<size_t>(tq) * ldqs + (weighted ? ldqs : 0)) * sizeof(float);
-    if (tq != 1 && tq != 2 && tq != 4) throw std::invalid_argument("pair scan supports 1, 2 or 4 queries per block");
+    if (classes ? (tq != 4 && tq != 8) : (tq != 1 && tq != 2 && tq != 4)) {
+        throw std::invalid_argument(classes ? "the chain-class pair scan supports 4 or 8 queries per block"
+                                            : "pair scan supports 1, 2 or 4 queries per block");
+    }
+    const bool cls = classes;
     // grid.y carries the query tile; batches beyond 65535 tiles go in slices.
     const uint32_t slice = 65535U * static_cast<uint32_t>(tq);
     for (uint32_t q0 = 0; q0 < nq; q0 += slice) {
@@ -332,16 +349,27 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
         const float* qs = queries + static_cast<uint64_t>(q0) * ld_q;
         unsigned long long* ks = keys + q0;
         const dim3 grid((rows + 255) / 256, (n + tq - 1) / tq);
-#define UHANN_PAIRS(M, W, F, Q)                                                                   \
-    hipLaunchKernelGGL((scan_pairs_kernel<M, W, F, Q>), grid, dim3(256), lds, stream, refs_t, ld_t, \
-                       rows, qs, ld_q, n, weights, dim, ks)
+#define UHANN_PAIRS(M, W, F, Q)                                                                       \
+    do {                                                                                              \
+        if constexpr (Q >= 4) {                                                                       \
+            if (cls) {                                                                                \
+                hipLaunchKernelGGL((scan_pairs_cls_kernel<M, W, F, Q>), grid, dim3(256), lds, stream, refs_t, \
+                                   ld_t, rows, qs, ld_q, n, weights, dim, ks);                        \
+                break;                                                                                \
+            }                                                                                         \
+        }                                                                                             \
+        if constexpr (Q <= 4) {                                                                       \
+            hipLaunchKernelGGL((scan_pairs_kernel<M, W, F, Q>), grid, dim3(256), lds, stream, refs_t,     \
+                               ld_t, rows, qs, ld_q, n, weights, dim, ks);                            \
+        }                                                                                             \
+    } while (0)
     const int m = metric == Metric::l2 ? 0 : 1;
-    if (tq == 1) {
+    if (tq == 1 && !cls) {
         if (m == 0) UHANN_PAIRS(0, false, false, 1);
         else if (!weighted) UHANN_PAIRS(1, false, false, 1);
         else if (weight_fma) UHANN_PAIRS(1, true, true, 1);
         else UHANN_PAIRS(1, true, false, 1);
-    } else if (tq == 2) {
+    } else if (tq == 2 && !cls) {
         if (m == 0) UHANN_PAIRS(0, false, false, 2);
         else if (!weighted) UHANN_PAIRS(1, false, false, 2);
         else if (weight_fma) UHANN_PAIRS(1, true, true, 2);
@@ -351,6 +379,11 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
         else if (!weighted) UHANN_PAIRS(1, false, false, 4);
         else if (weight_fma) UHANN_PAIRS(1, true, true, 4);
         else UHANN_PAIRS(1, true, false, 4);
+    } else {
+        if (m == 0) UHANN_PAIRS(0, false, false, 8);
+        else if (!weighted) UHANN_PAIRS(1, false, false, 8);
+        else if (weight_fma) UHANN_PAIRS(1, true, true, 8);
+        else UHANN_PAIRS(1, true, false, 8);
     }
 #undef UHANN_PAIRS
     check_launch("pair scan kernel");
