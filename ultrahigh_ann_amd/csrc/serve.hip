@@ -139,8 +139,9 @@ serve_pairs_kernel(const void* blob, int stage, unsigned long long* keys)
         const Unit w = locate(blob, stage, u);
         const ServeItem& it = *w.item;
         __syncthreads();
+        // a: query tile (the fast index), b: 256-row reference block
         body::scan_pairs_serve<METRIC, WEIGHTED, true, TQ>(it.refs_t, it.rows, it.rows, it.scan_q, it.ld_sq, it.n,
-                                                            it.weights, it.dim, keys + it.qoff, w.b * TQ, w.a, smem);
+                                                            it.weights, it.dim, keys + it.qoff, w.a * TQ, w.b, smem);
     }
 }
 

@@ -461,9 +461,12 @@ void uhann_server::launch(Slot& slot, int slot_index)
         it.rows = static_cast<uint32_t>(scan->rows);
         const bool l1 = scan->metric == UHANN_METRIC_L1;
         if (scan->refs_t.bytes() != 0 && it.dim != 0 && it.dim <= kServePairsMaxDim) {
+            // Query tiles fastest: the units that run together share a
+            // 256-row reference block, which then comes from L2.
             const uint64_t f = ceil_div(scan->rows, 256);
+            const uint64_t nb = ceil_div(r.n, static_cast<uint64_t>(cfg.pairs_tq));
             const int stage = !l1 ? kServePairsL2 : scan->weighted ? kServePairsL1W : kServePairsL1;
-            add(stage, idx, static_cast<uint32_t>(f), f * ceil_div(r.n, static_cast<uint64_t>(cfg.pairs_tq)));
+            add(stage, idx, static_cast<uint32_t>(nb), f * nb);
         } else {
             const uint64_t f = ceil_div(scan->rows, kServeTileR);
             const int stage = !l1 ? kServeTileL2 : scan->weighted ? kServeTileL1W : kServeTileL1;
