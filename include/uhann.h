@@ -535,7 +535,8 @@ typedef struct uhann_server_config {
     uint64_t scratch_floats;     /* per tick: gathered + projected queries */
     uint64_t staging_floats;     /* per tick: host queries copied to the device */
     int use_graph;               /* 1: replay captured graphs, 0: launch the kernels */
-    int pairs_tq;                /* queries per block of the pair-per-thread scans (1, 2, 4) */
+    int pairs_tq;                /* queries per block of the pair-per-thread scans (1, 2: whole
+                                    32-chain groups; 4: chain classes, the default) */
 } uhann_server_config;
 
 /* Index description for uhann_server_register. host_refs is borrowed until

@@ -507,7 +507,10 @@ __device__ __forceinline__ void pair_group_terms(float (&v)[TQ][32], const float
     } else {
 #pragma unroll
         for (int i = 0; i < 32; ++i) {
-            const uint32_t c = min(base + i, dim - 1U);
+            // base < dim: column 0 needs no clamp (a clamped offset is
+            // computed on the vector unit, and a vector soffset would cost
+            // a readfirstlane loop around the load)
+            const uint32_t c = i == 0 ? base : min(base + i, dim - 1U);
             rv[i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(cols, roff, (c - base) * col_bytes, 0));
         }
 #pragma unroll
@@ -573,7 +576,7 @@ __device__ __forceinline__ void pair_class(float (&cls)[TQ], const float* __rest
         const bool full = base + 28U < dim;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            const uint32_t c = full ? base + 4U * s : min(base + 4U * s, dim - 1U);
+            const uint32_t c = full || s == 0 ? base + 4U * s : min(base + 4U * s, dim - 1U);
             rv[s] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(cols, roff, (c - base) * col_bytes, 0));
         }
 #pragma unroll
@@ -603,6 +606,91 @@ __device__ __forceinline__ void pair_class(float (&cls)[TQ], const float* __rest
     }
 }
 
+// One unit of a chain class: columns base + 4s (s = 0..7) of the eight
+// chains, their reference values in rv; INIT starts the chains (the first
+// term onto +0). Columns past dim read zeros (padded reference and query
+// copies, weight 0) and add +0, which changes no bit of a non-negative sum.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, bool INIT>
+__device__ __forceinline__ void cls_unit(float (&v)[TQ][8], const float (&rv)[8], const float* qs, const float* ws,
+                                         uint32_t base)
+{
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const uint32_t c = base + 4U * s;
+        float qv[TQ];
+        lds_query_column<TQ>(qs, c, qv);
+        const float wv = WEIGHTED ? ws[c] : 0.0F;
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+            v[q][s] = accumulate_diff<METRIC, WEIGHTED, WFMA>(INIT ? 0.0F : v[q][s], rv[s] - qv[q], wv);
+    }
+}
+
+// One group's sum W by chain classes, visited in the order m = 0, 2, 1, 3:
+// W = (d0 + d2) + (d1 + d3), a class without columns (m >= dim - gbase)
+// being +0. Each unit's eight reference loads are issued one unit ahead,
+// across class boundaries too, so a wave has loads in flight while it adds.
+// refs_t holds zero rows past dim (a unit reads up to column dim + 27).
+// Needs gbase < dim.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+__device__ __forceinline__ void pair_group_cls(float (&gsum)[TQ], const float* __restrict__ refs_t, uint64_t ld_t,
+                                               uint32_t roff, const float* qs, const float* ws, uint32_t gbase,
+                                               uint32_t dim)
+{
+    const uint32_t col_bytes = static_cast<uint32_t>(ld_t) * 4U;
+    // Columns base + 4s: the resource sits at column base, column base + 4s
+    // is 4s rows of refs_t past it (rows < 2^25 keep the offsets below 2^32).
+    auto load_unit = [&](uint32_t base, float (&r)[8]) __attribute__((always_inline)) {
+        const __amdgpu_buffer_rsrc_t cols = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float*>(refs_t + static_cast<uint64_t>(base) * ld_t), 0, -1, kBufferWord3);
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            r[s] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(cols, roff, 4U * s * col_bytes, 0));
+    };
+    const uint32_t span = dim - gbase;  // columns from gbase on (>= 1)
+    float rn[8];
+    load_unit(gbase, rn);
+    // Class m (first column gbase + m < dim); the loads run one unit ahead,
+    // into `next` (the first column of the next class in the order that has
+    // columns, or dim for none) after the class's last unit.
+    auto run_class = [&](uint32_t m, uint32_t next, float (&d)[TQ]) __attribute__((always_inline)) {
+        float v[TQ][8];
+        uint32_t cur = gbase + m;
+        auto step = [&](auto init) __attribute__((always_inline)) {
+            float rv[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) rv[s] = rn[s];
+            const uint32_t nxt = cur + kChains < dim ? cur + kChains : next;
+            if (nxt < dim) load_unit(nxt, rn);
+            cls_unit<METRIC, WEIGHTED, WFMA, TQ, decltype(init)::value>(v, rv, qs, ws, cur);
+        };
+        step(BoolTag<true>{});
+        while (cur + kChains < dim) {
+            cur += kChains;
+            step(BoolTag<false>{});
+        }
+        // the class's part of the 32-lane tree (chain offsets 16, 8, 4)
+#pragma unroll
+        for (int q = 0; q < TQ; ++q) {
+#pragma unroll
+            for (int off = 4; off >= 1; off /= 2) {
+#pragma unroll
+                for (int s = 0; s < off; ++s) v[q][s] = v[q][s] + v[q][s + off];
+            }
+            d[q] = v[q][0];
+        }
+    };
+    float d0[TQ], d2[TQ], d1[TQ], d3[TQ];
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) d2[q] = d1[q] = d3[q] = 0.0F;
+    run_class(0U, span > 2U ? gbase + 2U : (span > 1U ? gbase + 1U : dim), d0);
+    if (span > 2U) run_class(2U, gbase + 1U, d2);
+    if (span > 1U) run_class(1U, span > 3U ? gbase + 3U : dim, d1);
+    if (span > 3U) run_class(3U, dim, d3);
+#pragma unroll
+    for (int q = 0; q < TQ; ++q) gsum[q] = (d0[q] + d2[q]) + (d1[q] + d3[q]);
+}
+
 // Pair-per-thread scan by chain classes (the default for short rows): each
 // thread owns one reference row and TQ <= 8 queries. A group of 32 chains is
 // taken a chain class at a time (chains 32w + m + 4s, m in the tree order
@@ -624,7 +712,7 @@ __device__ __forceinline__ void scan_pairs_cls(const float* __restrict__ refs_t,
     ld_t = uniform_u64(ld_t);
     rows = __builtin_amdgcn_readfirstlane(rows);
     dim = __builtin_amdgcn_readfirstlane(dim);
-    const uint32_t ldqs = (dim + 3U) & ~3U;
+    const uint32_t ldqs = (dim + 31U) & ~3U;  // zero columns up to dim + 27
     float* qs = smem;               // ldqs x TQ, column-major
     float* ws = smem + TQ * ldqs;   // ldqs (weighted L1)
     const int t = threadIdx.x;
@@ -660,16 +748,7 @@ __device__ __forceinline__ void scan_pairs_cls(const float* __restrict__ refs_t,
 #pragma unroll
                 for (int q = 0; q < TQ; ++q) gsum[q] = 0.0F;
             } else {
-                float e0[TQ], d[TQ];
-                pair_class<METRIC, WEIGHTED, WFMA, TQ>(e0, refs_t, ld_t, roff, qs, ws, gbase + 0U, dim);
-                pair_class<METRIC, WEIGHTED, WFMA, TQ>(d, refs_t, ld_t, roff, qs, ws, gbase + 2U, dim);
-#pragma unroll
-                for (int q = 0; q < TQ; ++q) e0[q] = e0[q] + d[q];
-                float e1[TQ];
-                pair_class<METRIC, WEIGHTED, WFMA, TQ>(e1, refs_t, ld_t, roff, qs, ws, gbase + 1U, dim);
-                pair_class<METRIC, WEIGHTED, WFMA, TQ>(d, refs_t, ld_t, roff, qs, ws, gbase + 3U, dim);
-#pragma unroll
-                for (int q = 0; q < TQ; ++q) gsum[q] = e0[q] + (e1[q] + d[q]);
+                pair_group_cls<METRIC, WEIGHTED, WFMA, TQ>(gsum, refs_t, ld_t, roff, qs, ws, gbase, dim);
             }
 #pragma unroll
             for (int q = 0; q < TQ; ++q) pair_sum[q] = h == 0 ? gsum[q] : pair_sum[q] + gsum[q];

@@ -99,10 +99,11 @@ Scan::Scan(int metric_value, const float* host_refs, uint64_t rows_value, uint64
         }
     }
     if (dim != 0 && dim <= kern::scan_pairs_max_dim() && rows < kern::kPairsMaxRows) {
-        // Short rows also get a transposed copy (dim x rows) for the
-        // pair-per-thread scan, whose loads are then contiguous across a wave
-        // (that scan takes fewer than 2^25 rows, so larger sets skip it).
-        std::vector<float> t(static_cast<std::size_t>(rows * dim));
+        // Short rows also get a transposed copy (dim x rows, then
+        // kPairsPadCols rows of zeros) for the pair-per-thread scan, whose
+        // loads are then contiguous across a wave (that scan takes fewer
+        // than 2^25 rows, so larger sets skip it).
+        std::vector<float> t(static_cast<std::size_t>(rows * (dim + kern::kPairsPadCols)), 0.0F);
         for (uint64_t r = 0; r < rows; ++r) {
             for (uint64_t c = 0; c < dim; ++c) t[c * rows + r] = host_refs[r * dim + c];
         }
@@ -169,8 +170,8 @@ Workspace::~Workspace()
     }
 }
 
-// Rows up to this length use the pair-per-thread scan (UHANN_PAIRS_MAX_DIM
-// overrides it for tuning).
+// Rows up to this length use the pair-per-thread scan by default (above it
+// the tiled kernel is faster; UHANN_PAIRS_MAX_DIM overrides it for tuning).
 uint32_t pairs_max_dim()
 {
     const char* env = std::getenv("UHANN_PAIRS_MAX_DIM");
@@ -178,7 +179,7 @@ uint32_t pairs_max_dim()
         const long v = std::strtol(env, nullptr, 10);
         if (v >= 0) return static_cast<uint32_t>(std::min<long>(v, kern::scan_pairs_max_dim()));
     }
-    return 4096;
+    return 2560;
 }
 
 int batch_bucket(uint64_t nq)
@@ -315,9 +316,10 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     } else if (forced_tile == nullptr && short_ok && short_blocks >= device_cus(scan.device)) {
         tile = kern::ScanTile{0, 0, 0, 0};
     } else if (forced_tile == nullptr && pairs_ok && a.dim <= pairs_max_dim() && count >= 4) {
-        // chain classes, 4 queries per thread, for rows up to 320 coordinates
-        // (faster there); whole groups, 2 queries per thread, above
-        tile = a.dim <= 320 ? kern::ScanTile{0, 4, 8, 0} : kern::ScanTile{0, 2, 0, 0};
+        // chain classes, 4 queries per thread (at r = 7028, batch 4096 the
+        // fastest pair form from 64 to 3527 coordinates; the tiled kernel
+        // overtakes it near 2500)
+        tile = kern::ScanTile{0, 4, 8, 0};
     }
     if (tile.rpt == 0) {
         const kern::Metric metric = scan.metric == UHANN_METRIC_L2 ? kern::Metric::l2 : kern::Metric::l1;

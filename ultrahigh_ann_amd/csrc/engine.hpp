@@ -60,7 +60,7 @@ struct Scan {
     uint64_t ld = 0;
     bool weighted = false;
     DeviceBuffer<float> refs;
-    DeviceBuffer<float> refs_t;  // dim x rows copy for short rows (pair-per-thread scan)
+    DeviceBuffer<float> refs_t;  // (dim + kPairsPadCols) x rows copy for short rows (pair-per-thread scan)
     DeviceBuffer<float> weights;
     uint64_t payload_bytes = 0;
     std::string device_name;

@@ -92,11 +92,13 @@ void scan_short(Metric metric, bool weight_fma, const float* refs, uint64_t ld_r
                 unsigned long long* keys, hipStream_t stream);
 
 // Pair-per-thread direct scan for dim <= scan_pairs_max_dim(): refs_t is the
-// transposed (dim x rows, row stride ld_t) reference matrix; tq queries per
-// block, rows < kPairsMaxRows. classes: the chain-class body (tq 4 or 8)
-// instead of whole 32-chain groups (tq 1, 2 or 4). Same keys contract as
-// scan_direct, same bits.
+// transposed reference matrix (row stride ld_t), dim rows followed by
+// kPairsPadCols rows of zeros; tq queries per block, rows < kPairsMaxRows.
+// classes: the chain-class body (tq 4 or 8; it reads up to 28 of the zero
+// rows, whose terms add +0) instead of whole 32-chain groups (tq 1, 2 or 4).
+// Same keys contract as scan_direct, same bits.
 constexpr uint32_t kPairsMaxRows = 1U << 25;
+constexpr uint32_t kPairsPadCols = 32;
 uint32_t scan_pairs_max_dim();
 void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld_t, uint32_t rows,
                 const float* queries, uint64_t ld_q, uint32_t nq, const float* weights, uint32_t dim,

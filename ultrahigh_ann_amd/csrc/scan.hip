@@ -335,7 +335,9 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     if (dim > scan_pairs_max_dim()) throw std::invalid_argument("pair scan supports dim <= 4096");
     const bool weighted = weights != nullptr;
     if (rows >= kPairsMaxRows) throw std::length_error("pair scan supports fewer than 2^25 reference rows");
-    const size_t ldqs = (static_cast<size_t>(dim) + 3) & ~static_cast<size_t>(3);
+    // query (and weight) columns in LDS: padded to a multiple of 4, and the
+    // chain-class body reads zero columns up to dim + 27
+    const size_t ldqs = (static_cast<size_t>(dim) + (classes ? 31 : 3)) & ~static_cast<size_t>(3);
     const size_t lds = (static_cast<size_t>(tq) * ldqs + (weighted ? ldqs : 0)) * sizeof(float);
     if (classes ? (tq != 4 && tq != 8) : (tq != 1 && tq != 2 && tq != 4)) {
         throw std::invalid_argument(classes ? "the chain-class pair scan supports 4 or 8 queries per block"

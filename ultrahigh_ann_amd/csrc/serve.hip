@@ -145,6 +145,24 @@ serve_pairs_kernel(const void* blob, int stage, unsigned long long* keys)
     }
 }
 
+// Pair stages at 4 queries per unit: the chain-class body (reference copies
+// carry kPairsPadCols zero rows past dim).
+template <int METRIC, bool WEIGHTED>
+__global__ void __launch_bounds__(kThreads)
+serve_pairs_cls_kernel(const void* blob, int stage, unsigned long long* keys)
+{
+    extern __shared__ float smem[];
+    const uint32_t total = header(blob).units[stage];
+    for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+        const Unit w = locate(blob, stage, u);
+        const ServeItem& it = *w.item;
+        __syncthreads();
+        // a: query tile (the fast index), b: 256-row reference block
+        body::scan_pairs_cls<METRIC, WEIGHTED, true, 4>(it.refs_t, it.rows, it.rows, it.scan_q, it.ld_sq, it.n,
+                                                        it.weights, it.dim, keys + it.qoff, w.a * 4U, w.b, smem);
+    }
+}
+
 template <int METRIC, bool WEIGHTED>
 __global__ void __launch_bounds__(kThreads)
 serve_tile_kernel(const void* blob, int stage, unsigned long long* keys)
@@ -187,13 +205,26 @@ template <int TQ>
 void launch_pairs(const ServeLaunch& l, int stage, hipStream_t stream)
 {
     const bool weighted = stage == kServePairsL1W;
-    const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * kServePairsMaxDim * sizeof(float);
-    if (weighted) {
-        serve_pairs_kernel<1, true, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
-    } else if (stage == kServePairsL1) {
-        serve_pairs_kernel<1, false, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+    if constexpr (TQ == 4) {
+        // query and weight columns up to kServePairsMaxDim + 27 (zeros past dim)
+        const size_t ldqs = (kServePairsMaxDim + 31U) & ~3U;
+        const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * ldqs * sizeof(float);
+        if (weighted) {
+            serve_pairs_cls_kernel<1, true><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        } else if (stage == kServePairsL1) {
+            serve_pairs_cls_kernel<1, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        } else {
+            serve_pairs_cls_kernel<0, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        }
     } else {
-        serve_pairs_kernel<0, false, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * kServePairsMaxDim * sizeof(float);
+        if (weighted) {
+            serve_pairs_kernel<1, true, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        } else if (stage == kServePairsL1) {
+            serve_pairs_kernel<1, false, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        } else {
+            serve_pairs_kernel<0, false, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        }
     }
 }
 

@@ -40,7 +40,8 @@ enum ServeStage : int {
 constexpr uint32_t kServeTileQ = 8;
 constexpr uint32_t kServeTileR = 8;
 // Pair stages take scanned dimensions up to this (queries and weights in
-// (pairs_tq + 1) * kServePairsMaxDim floats of LDS, at most 60 KB).
+// (pairs_tq + 1) * kServePairsMaxDim floats of LDS, 28 more columns at
+// pairs_tq 4 (the chain-class body); at most 62 KB).
 constexpr uint32_t kServePairsMaxDim = 3072;
 
 struct ServeItem {
@@ -97,7 +98,7 @@ struct ServeLaunch {
     int* flags = nullptr;
     uint32_t max_q = 0;
     uint32_t max_items = 0;
-    int pairs_tq = 2;
+    int pairs_tq = 4;
     uint32_t grid = 1024;
 };
 

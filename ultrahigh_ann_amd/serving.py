@@ -78,7 +78,7 @@ class Server:
     """uhann_server: resident indexes, request queue and tick worker."""
 
     def __init__(self, device=0, hbm_budget_bytes=64 << 30, max_tick_queries=8192, max_tick_requests=512,
-                 scratch_floats=64 << 20, staging_floats=16 << 20, use_graph=True, pairs_tq=2):
+                 scratch_floats=64 << 20, staging_floats=16 << 20, use_graph=True, pairs_tq=4):
         cfg = ServerConfig(device, hbm_budget_bytes, max_tick_queries, max_tick_requests, scratch_floats,
                            staging_floats, int(bool(use_graph)), pairs_tq)
         h = c_void_p()
