@@ -648,11 +648,41 @@ __device__ __forceinline__ void pair_group_cls(float (&gsum)[TQ], const float* _
             r[s] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(cols, roff, 4U * s * col_bytes, 0));
     };
     const uint32_t span = dim - gbase;  // columns from gbase on (>= 1)
+    if (span >= 4U && span <= kChains) {
+        // one unit per class: two reference buffers in turn, no copies
+        auto tree = [&](float (&v)[TQ][8], float (&d)[TQ]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) {
+#pragma unroll
+                for (int off = 4; off >= 1; off /= 2) {
+#pragma unroll
+                    for (int s = 0; s < off; ++s) v[q][s] = v[q][s] + v[q][s + off];
+                }
+                d[q] = v[q][0];
+            }
+        };
+        float ra[8], rb[8], v[TQ][8], d0[TQ], d2[TQ], d1[TQ], d3[TQ];
+        load_unit(gbase, ra);
+        load_unit(gbase + 2U, rb);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, ra, qs, ws, gbase);
+        tree(v, d0);
+        load_unit(gbase + 1U, ra);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, rb, qs, ws, gbase + 2U);
+        tree(v, d2);
+        load_unit(gbase + 3U, rb);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, ra, qs, ws, gbase + 1U);
+        tree(v, d1);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, rb, qs, ws, gbase + 3U);
+        tree(v, d3);
+#pragma unroll
+        for (int q = 0; q < TQ; ++q) gsum[q] = (d0[q] + d2[q]) + (d1[q] + d3[q]);
+        return;
+    }
     float rn[8];
     load_unit(gbase, rn);
-    // Class m (first column gbase + m < dim); the loads run one unit ahead,
-    // into `next` (the first column of the next class in the order that has
-    // columns, or dim for none) after the class's last unit.
+    // Class m (first column gbase + m < dim, its data in rn); the loads run
+    // one unit ahead, into `next` (the first column of the next class in the
+    // order that has columns, or dim for none) after the class's last unit.
     auto run_class = [&](uint32_t m, uint32_t next, float (&d)[TQ]) __attribute__((always_inline)) {
         float v[TQ][8];
         uint32_t cur = gbase + m;
@@ -708,6 +738,7 @@ __device__ __forceinline__ void scan_pairs_cls(const float* __restrict__ refs_t,
                                                unsigned long long* __restrict__ keys, uint32_t q0, uint32_t rblock,
                                                float* smem)
 {
+    static_assert(TQ % 4 == 0, "the chain-class body takes 4 or 8 queries per block");
     refs_t = reinterpret_cast<const float*>(uniform_u64(reinterpret_cast<uint64_t>(refs_t)));
     ld_t = uniform_u64(ld_t);
     rows = __builtin_amdgcn_readfirstlane(rows);
@@ -716,13 +747,17 @@ __device__ __forceinline__ void scan_pairs_cls(const float* __restrict__ refs_t,
     float* qs = smem;               // ldqs x TQ, column-major
     float* ws = smem + TQ * ldqs;   // ldqs (weighted L1)
     const int t = threadIdx.x;
-    for (uint32_t e = t; e < TQ * ldqs; e += 256) {
-        const uint32_t qi = e / ldqs;
-        const uint32_t c = e % ldqs;
-        qs[c * TQ + qi] = c < dim ? queries[static_cast<uint64_t>(min(q0 + qi, nq - 1U)) * ld_q + c] : 0.0F;
-    }
-    if constexpr (WEIGHTED) {
-        for (uint32_t c = t; c < ldqs; c += 256) ws[c] = c < dim ? weights[c] : 0.0F;
+    // a thread per column: TQ coalesced row reads, one or two 16-byte stores
+    for (uint32_t c = t; c < ldqs; c += 256) {
+        float x[TQ];
+#pragma unroll
+        for (int qi = 0; qi < TQ; ++qi)
+            x[qi] = c < dim ? queries[static_cast<uint64_t>(min(q0 + qi, nq - 1U)) * ld_q + c] : 0.0F;
+#pragma unroll
+        for (int k = 0; k < TQ / 4; ++k)
+            *reinterpret_cast<float4*>(qs + c * TQ + 4 * k) = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2],
+                                                                          x[4 * k + 3]);
+        if constexpr (WEIGHTED) ws[c] = c < dim ? weights[c] : 0.0F;
     }
     __syncthreads();
     const uint32_t r = rblock * 256 + t;
