@@ -449,8 +449,8 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
     od = torch.zeros(B, dtype=torch.float32, device="cuda")
     steps = 2
 
-    def add(name, fn):
-        if args.suite and name not in args.suite.split(","):
+    def add(name, fn, opt_in=False):
+        if (args.suite or opt_in) and name not in args.suite.split(","):
             return
         try:
             res[name] = fn()
@@ -504,6 +504,40 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
                                   "hbm_frac": round(bytes_alg / t / 8.0e12, 4),
                                   "kernel_ms": round(tk * 1e3, 4),
                                   "kernel_hbm_frac": round(bytes_alg / tk / 8.0e12, 4)}
+        return rows
+
+    def pair_tiles():
+        """Pair-per-thread scan tiles against the tiled kernel at the flat
+        index sample sizes (|I| = 222 / 917 / 3527 for L2, 65 / 287 / 1172 for
+        weighted L1; the leading columns of the bench rows), forced through
+        query_device_tile: 3 timed calls per tile after one untimed; every tile
+        must return the same bits. Named with --suite only (a profiling aid)."""
+        rows = {}
+        st = torch.cuda.current_stream()
+        for l1, dims in ((False, (222, 917, 3527)), (True, (65, 287, 1172))):
+            for dd in dims:
+                sub = np.ascontiguousarray(refs_np[:, :dd])
+                qd = queries[:, :dd].contiguous()
+                if l1:
+                    index = uh.HipDenseL1ScanIndex(sub, np.linspace(0.5, 2.0, dd).astype(np.float32))
+                else:
+                    index = uh.HipDenseL2ScanIndex(sub)
+                ws = index.make_query_workspace(B)
+                row, first = {}, None
+                for tile in ((0, 2, 0, 0), (0, 4, 8, 0), (0, 8, 8, 0), (1, 8, 8, 8)):
+                    ks = []
+                    for it in range(4):
+                        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        e0.record(st)
+                        index.query_device_tile(qd, B, ws, oi, od, tile, stream=st)
+                        e1.record(st)
+                        torch.cuda.synchronize()
+                        ks.append(e0.elapsed_time(e1))
+                    got = (oi.cpu().numpy().tobytes(), od.cpu().numpy().tobytes())
+                    first = got if first is None else first
+                    row[str(list(tile))] = {"ms": round(float(np.median(ks[1:])), 4), "same_bits": got == first}
+                rows[f"{'l1w' if l1 else 'l2'}_{dd}"] = row
+                del index, ws
         return rows
 
     probs = {}
@@ -577,6 +611,7 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
     add("flat_l1", lambda: flat(uh.METRIC_L1))
     add("hierarchical_l2_jl", lambda: hierarchical(uh.METRIC_L2))
     add("hierarchical_l1_cauchy", lambda: hierarchical(uh.METRIC_L1))
+    add("pair_tiles", pair_tiles, opt_in=True)
     return res
 
 
@@ -592,7 +627,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="queries timed on the CPU (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-suite", action="store_true", help="skip the secondary workloads")
-    ap.add_argument("--suite", default="", help="comma-separated secondary workloads to run (default: all)")
+    ap.add_argument("--suite", default="",
+                    help="comma-separated secondary workloads to run (default: all but pair_tiles)")
     ap.add_argument("--suite-only", action="store_true",
                     help="run only the secondary workloads (for profiling them); prints {\"workloads\": ...}")
     ap.add_argument("--no-poisson", action="store_true",
