@@ -38,6 +38,9 @@ CASES = [
     ("hier_l1", "hier", 1, 80, 1500, 16, 9, False, 21),
     ("hier_l1_p64", "hier", 1, 300, 1500, 24, 64, False, 7),
     ("hier_one_row", "hier", 0, 1, 1500, 4, 5, False, 3),
+    # about 1750 sampled coordinates: the weighted-L1 pair stage for rows
+    # longer than the short pair stages take
+    ("flat_l1_mid", "flat", 1, 40, 4000, 800, 0, False, 13),
 ]
 
 

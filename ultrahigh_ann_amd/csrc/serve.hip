@@ -204,23 +204,26 @@ serve_median_kernel(const void* blob, int stage, unsigned long long* keys)
 template <int TQ>
 void launch_pairs(const ServeLaunch& l, int stage, hipStream_t stream)
 {
-    const bool weighted = stage == kServePairsL1W;
+    const bool weighted = stage == kServePairsL1W || stage == kServePairsShortL1W;
+    const bool plain_l1 = stage == kServePairsL1 || stage == kServePairsShortL1;
+    const bool short_rows = stage == kServePairsShortL2 || stage == kServePairsShortL1 || stage == kServePairsShortL1W;
+    const uint32_t max_dim = short_rows ? kServePairsShortDim : kServePairsMaxDim;
     if constexpr (TQ == 4) {
-        // query and weight columns up to kServePairsMaxDim + 27 (zeros past dim)
-        const size_t ldqs = (kServePairsMaxDim + 31U) & ~3U;
+        // query and weight columns up to max_dim + 27 (zeros past dim)
+        const size_t ldqs = (max_dim + 31U) & ~3U;
         const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * ldqs * sizeof(float);
         if (weighted) {
             serve_pairs_cls_kernel<1, true><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
-        } else if (stage == kServePairsL1) {
+        } else if (plain_l1) {
             serve_pairs_cls_kernel<1, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
         } else {
             serve_pairs_cls_kernel<0, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
         }
     } else {
-        const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * kServePairsMaxDim * sizeof(float);
+        const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * max_dim * sizeof(float);
         if (weighted) {
             serve_pairs_kernel<1, true, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
-        } else if (stage == kServePairsL1) {
+        } else if (plain_l1) {
             serve_pairs_kernel<1, false, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
         } else {
             serve_pairs_kernel<0, false, TQ><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
@@ -243,7 +246,8 @@ void serve_tick(const ServeLaunch& l, uint32_t mask, hipStream_t stream)
     if (on(kServeGather)) serve_gather_kernel<<<l.grid, kThreads, 0, stream>>>(l.blob);
     if (on(kServeProject32)) serve_project32_kernel<<<l.grid, kThreads, 0, stream>>>(l.blob);
     if (on(kServeProject16)) serve_project16_kernel<<<l.grid, kThreads, 0, stream>>>(l.blob);
-    for (int s : {kServePairsL2, kServePairsL1, kServePairsL1W}) {
+    for (int s : {kServePairsShortL2, kServePairsShortL1, kServePairsShortL1W, kServePairsL2, kServePairsL1,
+                  kServePairsL1W}) {
         if (!on(s)) continue;
         if (l.pairs_tq == 1) {
             launch_pairs<1>(l, s, stream);

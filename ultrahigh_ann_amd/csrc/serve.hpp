@@ -23,6 +23,9 @@ enum ServeStage : int {
     kServeGather,      // sampled coordinates (unit = 1024-column chunk x 8 rows)
     kServeProject32,   // binary32 projection, K7 order (unit = 4 queries x 8 outputs)
     kServeProject16,   // bf16 MFMA projection (unit = 32 queries x 32 outputs)
+    kServePairsShortL2,   // pair-per-thread scans of rows up to kServePairsShortDim
+    kServePairsShortL1,   // (unit = 256 references x pairs_tq queries; less LDS per block)
+    kServePairsShortL1W,
     kServePairsL2,     // pair-per-thread scans (unit = 256 references x pairs_tq queries)
     kServePairsL1,
     kServePairsL1W,
@@ -41,8 +44,11 @@ constexpr uint32_t kServeTileQ = 8;
 constexpr uint32_t kServeTileR = 8;
 // Pair stages take scanned dimensions up to this (queries and weights in
 // (pairs_tq + 1) * kServePairsMaxDim floats of LDS, 28 more columns at
-// pairs_tq 4 (the chain-class body); at most 62 KB).
+// pairs_tq 4 (the chain-class body); at most 62 KB). Rows up to
+// kServePairsShortDim go to the short pair stages, whose blocks reserve LDS
+// for that many columns only, so more of them share a CU.
 constexpr uint32_t kServePairsMaxDim = 3072;
+constexpr uint32_t kServePairsShortDim = 1024;
 
 struct ServeItem {
     const float* queries;    // n full-width queries, row stride ld_q

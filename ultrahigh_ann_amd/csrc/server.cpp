@@ -465,7 +465,10 @@ void uhann_server::launch(Slot& slot, int slot_index)
             // 256-row reference block, which then comes from L2.
             const uint64_t f = ceil_div(scan->rows, 256);
             const uint64_t nb = ceil_div(r.n, static_cast<uint64_t>(cfg.pairs_tq));
-            const int stage = !l1 ? kServePairsL2 : scan->weighted ? kServePairsL1W : kServePairsL1;
+            const bool short_rows = it.dim <= kServePairsShortDim;
+            const int stage = !l1 ? (short_rows ? kServePairsShortL2 : kServePairsL2)
+                              : scan->weighted ? (short_rows ? kServePairsShortL1W : kServePairsL1W)
+                                               : (short_rows ? kServePairsShortL1 : kServePairsL1);
             add(stage, idx, static_cast<uint32_t>(nb), f * nb);
         } else {
             const uint64_t f = ceil_div(scan->rows, kServeTileR);
