@@ -607,16 +607,18 @@ __device__ __forceinline__ void pair_class(float (&cls)[TQ], const float* __rest
 }
 
 // One unit of a chain class: columns base + 4s (s = 0..7) of the eight
-// chains, their reference values in rv; INIT starts the chains (the first
-// term onto +0). Columns past dim read zeros (padded reference and query
-// copies, weight 0) and add +0, which changes no bit of a non-negative sum.
+// chains, their reference values in rv, their query values (and weights) at
+// local columns lbase + 4s of the group's LDS copy; INIT starts the chains
+// (the first term onto +0). Columns past dim read zeros (padded reference
+// and query copies, weight 0) and add +0, which changes no bit of a
+// non-negative sum.
 template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, bool INIT>
 __device__ __forceinline__ void cls_unit(float (&v)[TQ][8], const float (&rv)[8], const float* qs, const float* ws,
-                                         uint32_t base)
+                                         uint32_t lbase)
 {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const uint32_t c = base + 4U * s;
+        const uint32_t c = lbase + 4U * s;
         float qv[TQ];
         lds_query_column<TQ>(qs, c, qv);
         const float wv = WEIGHTED ? ws[c] : 0.0F;
@@ -630,13 +632,16 @@ __device__ __forceinline__ void cls_unit(float (&v)[TQ][8], const float (&rv)[8]
 // W = (d0 + d2) + (d1 + d3), a class without columns (m >= dim - gbase)
 // being +0. Each unit's eight reference loads are issued one unit ahead,
 // across class boundaries too, so a wave has loads in flight while it adds.
-// refs_t holds zero rows past dim (a unit reads up to column dim + 27).
+// refs_t holds zero rows past dim (a unit reads up to column dim + 27); qs
+// and ws hold column c at local column c (the whole row) or, LOCAL, column
+// gbase + j + 256 t at local column 32 t + j (the group's columns only).
 // Needs gbase < dim.
-template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, bool LOCAL>
 __device__ __forceinline__ void pair_group_cls(float (&gsum)[TQ], const float* __restrict__ refs_t, uint64_t ld_t,
                                                uint32_t roff, const float* qs, const float* ws, uint32_t gbase,
                                                uint32_t dim)
 {
+    const uint32_t l0 = LOCAL ? 0U : gbase;
     const uint32_t col_bytes = static_cast<uint32_t>(ld_t) * 4U;
     // Columns base + 4s: the resource sits at column base, column base + 4s
     // is 4s rows of refs_t past it (rows < 2^25 keep the offsets below 2^32).
@@ -664,15 +669,15 @@ __device__ __forceinline__ void pair_group_cls(float (&gsum)[TQ], const float* _
         float ra[8], rb[8], v[TQ][8], d0[TQ], d2[TQ], d1[TQ], d3[TQ];
         load_unit(gbase, ra);
         load_unit(gbase + 2U, rb);
-        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, ra, qs, ws, gbase);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, ra, qs, ws, l0);
         tree(v, d0);
         load_unit(gbase + 1U, ra);
-        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, rb, qs, ws, gbase + 2U);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, rb, qs, ws, l0 + 2U);
         tree(v, d2);
         load_unit(gbase + 3U, rb);
-        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, ra, qs, ws, gbase + 1U);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, ra, qs, ws, l0 + 1U);
         tree(v, d1);
-        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, rb, qs, ws, gbase + 3U);
+        cls_unit<METRIC, WEIGHTED, WFMA, TQ, true>(v, rb, qs, ws, l0 + 3U);
         tree(v, d3);
 #pragma unroll
         for (int q = 0; q < TQ; ++q) gsum[q] = (d0[q] + d2[q]) + (d1[q] + d3[q]);
@@ -685,18 +690,19 @@ __device__ __forceinline__ void pair_group_cls(float (&gsum)[TQ], const float* _
     // order that has columns, or dim for none) after the class's last unit.
     auto run_class = [&](uint32_t m, uint32_t next, float (&d)[TQ]) __attribute__((always_inline)) {
         float v[TQ][8];
-        uint32_t cur = gbase + m;
+        uint32_t cur = gbase + m, lcur = m;
         auto step = [&](auto init) __attribute__((always_inline)) {
             float rv[8];
 #pragma unroll
             for (int s = 0; s < 8; ++s) rv[s] = rn[s];
             const uint32_t nxt = cur + kChains < dim ? cur + kChains : next;
             if (nxt < dim) load_unit(nxt, rn);
-            cls_unit<METRIC, WEIGHTED, WFMA, TQ, decltype(init)::value>(v, rv, qs, ws, cur);
+            cls_unit<METRIC, WEIGHTED, WFMA, TQ, decltype(init)::value>(v, rv, qs, ws, LOCAL ? lcur : cur);
         };
         step(BoolTag<true>{});
         while (cur + kChains < dim) {
             cur += kChains;
+            if constexpr (LOCAL) lcur += 32U;
             step(BoolTag<false>{});
         }
         // the class's part of the 32-lane tree (chain offsets 16, 8, 4)
@@ -730,8 +736,20 @@ __device__ __forceinline__ void pair_group_cls(float (&gsum)[TQ], const float* _
 // (W_w = (d0 + d2) + (d1 + d3) with d_m the class sums, then
 // ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7))), so the bits are the reference's.
 // Queries (and L1 weights) sit in LDS column-major (the TQ values of a
-// column together, read as broadcast vectors).
-template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+// column together, read as broadcast vectors), zeros past dim. WHOLE: the
+// whole row once (rows up to kClsWholeRowDim coordinates); otherwise a
+// pass's two groups at a time (group w: columns 32w + j + 256 t at local
+// column 32 t + j), which keeps the block's LDS small enough for several
+// blocks per CU. A block needs cls_lds_floats(TQ, WEIGHTED, dim) floats.
+constexpr uint32_t kClsWholeRowDim = 1536;
+__host__ __device__ constexpr uint32_t cls_lds_floats(int tq, bool weighted, uint32_t dim)
+{
+    return dim <= kClsWholeRowDim
+               ? static_cast<uint32_t>(tq + (weighted ? 1 : 0)) * ((dim + 31U) & ~3U)
+               : 2U * static_cast<uint32_t>(tq + (weighted ? 1 : 0)) * 32U * ((dim + 255U) / 256U);
+}
+
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, bool WHOLE>
 __device__ __forceinline__ void scan_pairs_cls(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
                                                const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
                                                const float* __restrict__ weights, uint32_t dim,
@@ -743,55 +761,108 @@ __device__ __forceinline__ void scan_pairs_cls(const float* __restrict__ refs_t,
     ld_t = uniform_u64(ld_t);
     rows = __builtin_amdgcn_readfirstlane(rows);
     dim = __builtin_amdgcn_readfirstlane(dim);
-    const uint32_t ldqs = (dim + 31U) & ~3U;  // zero columns up to dim + 27
-    float* qs = smem;               // ldqs x TQ, column-major
-    float* ws = smem + TQ * ldqs;   // ldqs (weighted L1)
     const int t = threadIdx.x;
-    // a thread per column: TQ coalesced row reads, one or two 16-byte stores
-    for (uint32_t c = t; c < ldqs; c += 256) {
+    // one column's TQ query values (coalesced row reads across the block)
+    // as one or two 16-byte stores, and its weight
+    auto stage_column = [&](uint32_t c, float* qs, float* ws) __attribute__((always_inline)) {
         float x[TQ];
 #pragma unroll
         for (int qi = 0; qi < TQ; ++qi)
             x[qi] = c < dim ? queries[static_cast<uint64_t>(min(q0 + qi, nq - 1U)) * ld_q + c] : 0.0F;
 #pragma unroll
         for (int k = 0; k < TQ / 4; ++k)
-            *reinterpret_cast<float4*>(qs + c * TQ + 4 * k) = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2],
-                                                                          x[4 * k + 3]);
-        if constexpr (WEIGHTED) ws[c] = c < dim ? weights[c] : 0.0F;
-    }
-    __syncthreads();
-    const uint32_t r = rblock * 256 + t;
-    const uint32_t roff = min(r, rows - 1U) * 4U;
+            *reinterpret_cast<float4*>(qs + 4 * k) = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+        if constexpr (WEIGHTED) *ws = c < dim ? weights[c] : 0.0F;
+    };
     // Groups in bit-reversed order 0,4 | 2,6 | 1,5 | 3,7: pass p produces
     // (W0+W4), (W2+W6), (W1+W5), (W3+W7) in turn, and the distance is
     // (P0 + P1) + (P2 + P3).
     float P[TQ][4];
-#pragma unroll
-    for (int q = 0; q < TQ; ++q)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) P[q][k] = 0.0F;
-#pragma unroll 1
-    for (int pass = 0; pass < 4; ++pass) {
-        float pair_sum[TQ];
-#pragma unroll 1
-        for (int h = 0; h < 2; ++h) {
-            const int g = 2 * pass + h;
-            const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
-            const uint32_t gbase = static_cast<uint32_t>(32 * w);
-            float gsum[TQ];
-            if (gbase >= dim) {
-#pragma unroll
-                for (int q = 0; q < TQ; ++q) gsum[q] = 0.0F;
-            } else {
-                pair_group_cls<METRIC, WEIGHTED, WFMA, TQ>(gsum, refs_t, ld_t, roff, qs, ws, gbase, dim);
-            }
-#pragma unroll
-            for (int q = 0; q < TQ; ++q) pair_sum[q] = h == 0 ? gsum[q] : pair_sum[q] + gsum[q];
-        }
+    uint32_t r;
+    if constexpr (WHOLE) {
+        const uint32_t ldqs = (dim + 31U) & ~3U;  // zero columns up to dim + 27
+        float* qs = smem;               // ldqs x TQ, column-major
+        float* ws = smem + TQ * ldqs;   // ldqs (weighted L1)
+        for (uint32_t c = t; c < ldqs; c += 256) stage_column(c, qs + c * TQ, ws + c);
+        __syncthreads();
+        r = rblock * 256 + t;
+        const uint32_t roff = min(r, rows - 1U) * 4U;
 #pragma unroll
         for (int q = 0; q < TQ; ++q)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) P[q][k] = pass == k ? pair_sum[q] : P[q][k];
+            for (int k = 0; k < 4; ++k) P[q][k] = 0.0F;
+#pragma unroll 1
+        for (int pass = 0; pass < 4; ++pass) {
+            float pair_sum[TQ];
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                const int g = 2 * pass + h;
+                const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
+                const uint32_t gbase = static_cast<uint32_t>(32 * w);
+                float gsum[TQ];
+                if (gbase >= dim) {
+#pragma unroll
+                    for (int q = 0; q < TQ; ++q) gsum[q] = 0.0F;
+                } else {
+                    pair_group_cls<METRIC, WEIGHTED, WFMA, TQ, false>(gsum, refs_t, ld_t, roff, qs, ws, gbase, dim);
+                }
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) pair_sum[q] = h == 0 ? gsum[q] : pair_sum[q] + gsum[q];
+            }
+#pragma unroll
+            for (int q = 0; q < TQ; ++q)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) P[q][k] = pass == k ? pair_sum[q] : P[q][k];
+        }
+    } else {
+        // two slots of gq floats (group w: column 32w + j + 256t at local
+        // column 32t + j), the weights after them
+        const uint32_t tmax = (dim + 255U) / 256U;  // terms of chain 0
+        const uint32_t gq = TQ * 32U * tmax;
+        float* wsbase = smem + 2U * gq;
+        r = rblock * 256 + t;
+        const uint32_t roff = min(r, rows - 1U) * 4U;
+#pragma unroll
+        for (int q = 0; q < TQ; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) P[q][k] = 0.0F;
+#pragma unroll 1
+        for (int pass = 0; pass < 4; ++pass) {
+            // the pass's two groups (after the previous pass's reads)
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int g = 2 * pass + h;
+                const uint32_t w = static_cast<uint32_t>(((g & 1) << 2) | (g & 2) | ((g & 4) >> 2));
+                if (32U * w >= dim) continue;
+                const uint32_t n = 32U * ((dim - 32U * w + 255U) / 256U);
+                for (uint32_t e = t; e < n; e += 256)
+                    stage_column(32U * w + (e & 31U) + 256U * (e >> 5), smem + h * gq + e * TQ,
+                                 wsbase + h * 32U * tmax + e);
+            }
+            __syncthreads();
+            float pair_sum[TQ];
+#pragma unroll 1
+            for (int h = 0; h < 2; ++h) {
+                const int g = 2 * pass + h;
+                const int w = ((g & 1) << 2) | (g & 2) | ((g & 4) >> 2);  // bit reversal of g
+                const uint32_t gbase = static_cast<uint32_t>(32 * w);
+                float gsum[TQ];
+                if (gbase >= dim) {
+#pragma unroll
+                    for (int q = 0; q < TQ; ++q) gsum[q] = 0.0F;
+                } else {
+                    pair_group_cls<METRIC, WEIGHTED, WFMA, TQ, true>(gsum, refs_t, ld_t, roff, smem + h * gq,
+                                                                     wsbase + h * 32U * tmax, gbase, dim);
+                }
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) pair_sum[q] = h == 0 ? gsum[q] : pair_sum[q] + gsum[q];
+            }
+#pragma unroll
+            for (int q = 0; q < TQ; ++q)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) P[q][k] = pass == k ? pair_sum[q] : P[q][k];
+        }
     }
     __shared__ unsigned long long best[TQ][4];
 #pragma unroll

@@ -170,8 +170,8 @@ Workspace::~Workspace()
     }
 }
 
-// Rows up to this length use the pair-per-thread scan by default (above it
-// the tiled kernel is faster; UHANN_PAIRS_MAX_DIM overrides it for tuning).
+// Rows up to this length use the pair-per-thread scan by default
+// (UHANN_PAIRS_MAX_DIM overrides it for tuning).
 uint32_t pairs_max_dim()
 {
     const char* env = std::getenv("UHANN_PAIRS_MAX_DIM");
@@ -179,7 +179,7 @@ uint32_t pairs_max_dim()
         const long v = std::strtol(env, nullptr, 10);
         if (v >= 0) return static_cast<uint32_t>(std::min<long>(v, kern::scan_pairs_max_dim()));
     }
-    return 2560;
+    return 4096;
 }
 
 int batch_bucket(uint64_t nq)
@@ -317,8 +317,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
         tile = kern::ScanTile{0, 0, 0, 0};
     } else if (forced_tile == nullptr && pairs_ok && a.dim <= pairs_max_dim() && count >= 4) {
         // chain classes, 4 queries per thread (at r = 7028, batch 4096 the
-        // fastest pair form from 64 to 3527 coordinates; the tiled kernel
-        // overtakes it near 2500)
+        // fastest form from 64 to 3527 coordinates, the tiled kernel included)
         tile = kern::ScanTile{0, 4, 8, 0};
     }
     if (tile.rpt == 0) {

@@ -83,16 +83,17 @@ scan_pairs_kernel(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows
 }
 
 // Chain-class form of the pair-per-thread scan (body::scan_pairs_cls): grid
-// as scan_pairs_kernel.
-template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
+// as scan_pairs_kernel; WHOLE (dim <= body::kClsWholeRowDim) stages the whole
+// query rows in LDS, otherwise two groups' columns per pass.
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, bool WHOLE>
 __global__ void __launch_bounds__(256)
 scan_pairs_cls_kernel(const float* __restrict__ refs_t, uint64_t ld_t, uint32_t rows,
                       const float* __restrict__ queries, uint64_t ld_q, uint32_t nq,
                       const float* __restrict__ weights, uint32_t dim, unsigned long long* __restrict__ keys)
 {
     extern __shared__ float smem[];
-    body::scan_pairs_cls<METRIC, WEIGHTED, WFMA, TQ>(refs_t, ld_t, rows, queries, ld_q, nq, weights, dim, keys,
-                                                     blockIdx.y * TQ, blockIdx.x, smem);
+    body::scan_pairs_cls<METRIC, WEIGHTED, WFMA, TQ, WHOLE>(refs_t, ld_t, rows, queries, ld_q, nq, weights, dim,
+                                                            keys, blockIdx.y * TQ, blockIdx.x, smem);
 }
 
 // Device-side escalation of the mixed-precision strategy (no host round
@@ -335,10 +336,11 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     if (dim > scan_pairs_max_dim()) throw std::invalid_argument("pair scan supports dim <= 4096");
     const bool weighted = weights != nullptr;
     if (rows >= kPairsMaxRows) throw std::length_error("pair scan supports fewer than 2^25 reference rows");
-    // query (and weight) columns in LDS: padded to a multiple of 4, and the
-    // chain-class body reads zero columns up to dim + 27
-    const size_t ldqs = (static_cast<size_t>(dim) + (classes ? 31 : 3)) & ~static_cast<size_t>(3);
-    const size_t lds = (static_cast<size_t>(tq) * ldqs + (weighted ? ldqs : 0)) * sizeof(float);
+    // query (and weight) columns in LDS: the whole row padded to a multiple
+    // of 4, or two groups' columns at a time for the chain-class body
+    const size_t ldqs = (static_cast<size_t>(dim) + 3) & ~static_cast<size_t>(3);
+    const size_t lds = classes ? body::cls_lds_floats(tq, weighted, dim) * sizeof(float)
+                               : (static_cast<size_t>(tq) * ldqs + (weighted ? ldqs : 0)) * sizeof(float);
     if (classes ? (tq != 4 && tq != 8) : (tq != 1 && tq != 2 && tq != 4)) {
         throw std::invalid_argument(classes ? "the chain-class pair scan supports 4 or 8 queries per block"
                                             : "pair scan supports 1, 2 or 4 queries per block");
@@ -355,8 +357,12 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     do {                                                                                              \
         if constexpr (Q >= 4) {                                                                       \
             if (cls) {                                                                                \
-                hipLaunchKernelGGL((scan_pairs_cls_kernel<M, W, F, Q>), grid, dim3(256), lds, stream, refs_t, \
-                                   ld_t, rows, qs, ld_q, n, weights, dim, ks);                        \
+                if (dim <= body::kClsWholeRowDim)                                                     \
+                    hipLaunchKernelGGL((scan_pairs_cls_kernel<M, W, F, Q, true>), grid, dim3(256), lds, stream, \
+                                       refs_t, ld_t, rows, qs, ld_q, n, weights, dim, ks);            \
+                else                                                                                  \
+                    hipLaunchKernelGGL((scan_pairs_cls_kernel<M, W, F, Q, false>), grid, dim3(256), lds, stream, \
+                                       refs_t, ld_t, rows, qs, ld_q, n, weights, dim, ks);            \
                 break;                                                                                \
             }                                                                                         \
         }                                                                                             \

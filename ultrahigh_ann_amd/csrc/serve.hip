@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -146,9 +147,10 @@ serve_pairs_kernel(const void* blob, int stage, unsigned long long* keys)
 }
 
 // Pair stages at 4 queries per unit: the chain-class body (reference copies
-// carry kPairsPadCols zero rows past dim).
-template <int METRIC, bool WEIGHTED>
-__global__ void __launch_bounds__(kThreads)
+// carry kPairsPadCols zero rows past dim); WHOLE for the short pair stages
+// (rows up to kServePairsShortDim <= body::kClsWholeRowDim).
+template <int METRIC, bool WEIGHTED, bool WHOLE>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
 serve_pairs_cls_kernel(const void* blob, int stage, unsigned long long* keys)
 {
     extern __shared__ float smem[];
@@ -158,8 +160,9 @@ serve_pairs_cls_kernel(const void* blob, int stage, unsigned long long* keys)
         const ServeItem& it = *w.item;
         __syncthreads();
         // a: query tile (the fast index), b: 256-row reference block
-        body::scan_pairs_cls<METRIC, WEIGHTED, true, 4>(it.refs_t, it.rows, it.rows, it.scan_q, it.ld_sq, it.n,
-                                                        it.weights, it.dim, keys + it.qoff, w.a * 4U, w.b, smem);
+        body::scan_pairs_cls<METRIC, WEIGHTED, true, 4, WHOLE>(it.refs_t, it.rows, it.rows, it.scan_q, it.ld_sq, it.n,
+                                                               it.weights, it.dim, keys + it.qoff, w.a * 4U, w.b,
+                                                               smem);
     }
 }
 
@@ -209,15 +212,23 @@ void launch_pairs(const ServeLaunch& l, int stage, hipStream_t stream)
     const bool short_rows = stage == kServePairsShortL2 || stage == kServePairsShortL1 || stage == kServePairsShortL1W;
     const uint32_t max_dim = short_rows ? kServePairsShortDim : kServePairsMaxDim;
     if constexpr (TQ == 4) {
-        // query and weight columns up to max_dim + 27 (zeros past dim)
-        const size_t ldqs = (max_dim + 31U) & ~3U;
-        const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * ldqs * sizeof(float);
-        if (weighted) {
-            serve_pairs_cls_kernel<1, true><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+        static_assert(kServePairsShortDim <= body::kClsWholeRowDim && kServePairsMaxDim > body::kClsWholeRowDim,
+                      "short pair stages stage whole rows, long ones two groups at a time");
+        const size_t lds = body::cls_lds_floats(TQ, weighted, max_dim) * sizeof(float);
+        if (short_rows) {
+            if (weighted) {
+                serve_pairs_cls_kernel<1, true, true><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+            } else if (plain_l1) {
+                serve_pairs_cls_kernel<1, false, true><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+            } else {
+                serve_pairs_cls_kernel<0, false, true><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+            }
+        } else if (weighted) {
+            serve_pairs_cls_kernel<1, true, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
         } else if (plain_l1) {
-            serve_pairs_cls_kernel<1, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+            serve_pairs_cls_kernel<1, false, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
         } else {
-            serve_pairs_cls_kernel<0, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
+            serve_pairs_cls_kernel<0, false, false><<<l.grid, kThreads, lds, stream>>>(l.blob, stage, l.keys);
         }
     } else {
         const size_t lds = static_cast<size_t>(TQ + (weighted ? 1 : 0)) * max_dim * sizeof(float);
