@@ -504,6 +504,29 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
                                   "hbm_frac": round(bytes_alg / t / 8.0e12, 4),
                                   "kernel_ms": round(tk * 1e3, 4),
                                   "kernel_hbm_frac": round(bytes_alg / tk / 8.0e12, 4)}
+        # The certified INT8 strategy at the same batch sizes: it streams the
+        # references' int8 planes (half the binary32 bytes) and returns the
+        # binary32 path's rows and bits.
+        index.enable_mixed(uh.STRATEGY_GRAM_INT8)
+        for b in (1, 4, 16):
+            index.query_device_async(queries, b, ws, oi, od)
+            want = (oi[:b].cpu().numpy().copy(), od[:b].cpu().numpy().view(np.uint32).copy())
+            wall, ms = timed(lambda: index.query_device_async(queries, b, ws, oi, od, strategy=uh.STRATEGY_GRAM_INT8),
+                             5, 2, torch)
+            t = float(np.median(ms)) * 1e-3
+            same = bool(np.array_equal(oi[:b].cpu().numpy(), want[0])
+                        and np.array_equal(od[:b].cpu().numpy().view(np.uint32), want[1]))
+            ws.set_timing(True)
+            index.query_device_async(queries, b, ws, oi, od, strategy=uh.STRATEGY_GRAM_INT8)
+            stage = [round(float(x), 4) for x in ws.stage_times()]
+            ws.set_timing(False)
+            rows[f"batch_{b}"]["int8_certified"] = {
+                "queries_per_s": round(b / t, 1), "ms": round(t * 1e3, 4),
+                "speedup_vs_fp32": round(rows[f"batch_{b}"]["ms"] / (t * 1e3), 2), "same_as_fp32": same,
+                "certified_fraction": round(ws.shortlist_stats()["certified"] / b, 4) if b else None,
+                "stage_ms": {"convert": stage[0], "gram": stage[1], "select_rerank": stage[2],
+                             "escalation": stage[3]},
+                "int8_planes_hbm_frac": round(2.0 * r * d / (stage[1] * 1e-3) / 8.0e12, 4)}
         return rows
 
     def pair_tiles():

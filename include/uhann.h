@@ -197,15 +197,16 @@ int uhann_scan_enable_mixed(uhann_scan* scan, int strategy);
  * reference (its unrefined SGEMM keeps no shortlist). */
 int uhann_scan_set_fp8_shortlist(uhann_scan* scan, uint32_t rows);
 
-/* Counters of the last GRAM_BF16 batch on this workspace: total candidate
- * set size, queries decided in binary64, queries escalated to the direct
- * scan. Synchronises the workspace stream. */
-/* Of the last GRAM_FP8 batch on this workspace: queries whose answer the FP8
- * error bound proves equal to the direct path's (no row outside the shortlist
- * can be closer). Synchronises the workspace stream. No counterpart in the
+/* Of the last GRAM_FP8 or GRAM_INT8 batch on this workspace: queries whose
+ * answer the strategy's error bound proves equal to the direct path's (no row
+ * outside the candidates can be closer; every query under GRAM_INT8 unless it
+ * escalated). Synchronises the workspace stream. No counterpart in the
  * reference. */
 int uhann_workspace_shortlist_stats(uhann_workspace* ws, uint64_t* certified);
 
+/* Counters of the last GRAM_BF16 or GRAM_INT8 batch on this workspace: total
+ * candidate set size, queries decided in binary64, queries escalated to the
+ * direct scan. Synchronises the workspace stream. */
 int uhann_workspace_mixed_stats(uhann_workspace* ws, uint64_t* candidates, uint64_t* fp64_reranked,
                                 uint64_t* escalated);
 
@@ -472,8 +473,9 @@ int uhann_scan_fp8_estimates(const uhann_scan* scan, uhann_workspace* ws, const 
 /* The GRAM_INT8 strategy's estimates ||q||^2 + ||v||^2 - 2 s_q s_v (H + X /
  * 256) (exact int32 sums, binary64 combination, rounded to binary32) for
  * count device queries against every reference, as uhann_scan_gram_estimates
- * (tile -1: the default shape with its K split; 0 .. 5 force a block shape,
- * every tile whole; all give the same bits, since the sums are exact). Needs
+ * (tile -1: the default shape for the batch with its K split; 0 .. 7 force a
+ * block shape, every tile whole; all give the same bits, since the sums are
+ * exact). Needs
  * uhann_scan_enable_mixed(GRAM_INT8). No counterpart in the reference: for
  * diagnostics and tests. */
 int uhann_scan_int8_estimates(const uhann_scan* scan, uhann_workspace* ws, const float* device_queries,

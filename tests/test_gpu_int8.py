@@ -112,7 +112,7 @@ def test_int8_estimates_bit_exact_and_bounded(uh, r, d, b):
     ws = index.make_query_workspace(b)
     dq = torch.from_numpy(q).cuda()
     outs = []
-    for tile in (-1, 0, 1, 2, 3, 4, 5):
+    for tile in (-1, 0, 1, 2, 3, 4, 5, 6, 7):
         out = torch.full((b, r + 3), float("nan"), dtype=torch.float32, device="cuda")
         index.int8_estimates(dq, b, ws, out, tile)
         torch.cuda.synchronize()

@@ -177,7 +177,7 @@ class QueryWorkspace:
         return [float(x) for x in ms]
 
     def shortlist_stats(self) -> dict:
-        """Certified queries of the last GRAM_FP8 batch."""
+        """Certified queries of the last GRAM_FP8 or GRAM_INT8 batch."""
         c = c_uint64()
         check(_native.lib().uhann_workspace_shortlist_stats(self._h, ctypes.byref(c)))
         return {"certified": c.value}

@@ -2118,6 +2118,12 @@ namespace {
 //   3: 4x4 waves, 1x2 tiles (128 x 256, 16 waves)
 //   4: 2x2 waves, 2x2 tiles (128 x 128, 4 waves: two blocks per CU)
 //   5: ping-pong 256 x 128 (gram_i8x2_pp_kernel; the 4x2 wave grid of shape 0)
+//   6: 1x4 waves, 1x1 tiles (32 x 128, small batches: up to 32 queries)
+//   7: 1x4 waves, 2x1 tiles (64 x 128, up to 64 queries)
+// Small batches read every reference row once whatever the tile height, so
+// a 256-row tile would spend its MFMAs on padding rows; the 32- and 64-row
+// shapes leave the scan bound by the int8 planes' HBM traffic, with the K
+// split spreading their few tiles over every CU.
 // group_m: tile rows swept together (tile_origin). Each XCD runs 32
 // consecutive tiles at once; four rows of 256 x 128 tiles touch
 // 4 x 256 + 8 x 128 operand rows per K step, eight rows 8 x 256 + 4 x 128.
@@ -2125,22 +2131,27 @@ struct I8Shape {
     int wm, wn, tm, tn;
     uint32_t group_m;
 };
-constexpr I8Shape kI8Shapes[6] = {{4, 2, 2, 2, 4}, {4, 4, 2, 1, 8}, {2, 4, 2, 2, 8},
-                                   {4, 4, 1, 2, 8}, {2, 2, 2, 2, 8}, {4, 2, 2, 2, 4}};
+constexpr I8Shape kI8Shapes[8] = {{4, 2, 2, 2, 4}, {4, 4, 2, 1, 8}, {2, 4, 2, 2, 8}, {4, 4, 1, 2, 8},
+                                   {2, 2, 2, 2, 8}, {4, 2, 2, 2, 4}, {1, 4, 1, 1, 1}, {1, 4, 2, 1, 1}};
 constexpr int kI8DefaultTile = 5;
+
+// The default shape for a batch of m queries.
+int i8_default_tile(uint32_t m) { return m <= 32 ? 6 : m <= 64 ? 7 : kI8DefaultTile; }
 
 }  // namespace
 
 uint64_t gram_i8_split_floats(uint32_t max_m, uint32_t n, uint32_t k_steps, uint32_t cus)
 {
-    const I8Shape sh = kI8Shapes[kI8DefaultTile];
-    const uint32_t bm = 32U * sh.tm * sh.wm;
-    const uint32_t bn = 32U * sh.tn * sh.wn;
-    const uint32_t tiles_n = (n + bn - 1) / bn;
+    // every batch size up to max_m with its default shape
     uint64_t need = 0;
-    for (uint32_t tm = 1; tm <= (max_m + bm - 1) / bm; ++tm) {
-        const GramSplit sp = gram_split_plan(tm * tiles_n, k_steps, cus);
+    for (uint32_t m = 1; m <= max_m;) {
+        const I8Shape sh = kI8Shapes[i8_default_tile(m)];
+        const uint32_t bm = 32U * sh.tm * sh.wm;
+        const uint32_t bn = 32U * sh.tn * sh.wn;
+        const uint32_t tiles_n = (n + bn - 1) / bn;
+        const GramSplit sp = gram_split_plan(((m + bm - 1) / bm) * tiles_n, k_steps, cus);
         need = std::max<uint64_t>(need, static_cast<uint64_t>(sp.tail) * sp.parts * 2U * bm * bn);
+        m = m <= 32 ? 33 : m <= 64 ? 65 : m + bm;  // next tile count (or shape)
     }
     return need;
 }
@@ -2154,7 +2165,8 @@ void gram_i8x2(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uin
     if (k / 2 > kI8MaxDim) throw std::length_error("int8 Gram rows are limited to 65536 columns");
     if (ldq % 16 != 0 || ldv % 16 != 0) throw std::invalid_argument("int8 Gram rows must be 16-byte aligned");
     if (tile < -1 || tile >= gram_i8_tile_count()) throw std::invalid_argument("unknown int8 Gram tile");
-    const I8Shape sh = kI8Shapes[tile < 0 ? kI8DefaultTile : tile];
+    const int shape = tile < 0 ? i8_default_tile(m) : tile;
+    const I8Shape sh = kI8Shapes[shape];
     const uint32_t bm = 32U * sh.tm * sh.wm;
     const uint32_t bn = 32U * sh.tn * sh.wn;
     const uint32_t tiles_m = (m + bm - 1) / bm;
@@ -2174,12 +2186,14 @@ void gram_i8x2(const uint8_t* q, uint64_t ldq, uint32_t m, const uint8_t* v, uin
                                                                             tiles_m, tiles_n, group_m);    \
         }                                                                                                  \
         break;
-    switch (tile < 0 ? kI8DefaultTile : tile) {
+    switch (shape) {
         UHANN_I8_CASE(0, 4, 2, 2, 2)
         UHANN_I8_CASE(1, 4, 4, 2, 1)
         UHANN_I8_CASE(2, 2, 4, 2, 2)
         UHANN_I8_CASE(3, 4, 4, 1, 2)
         UHANN_I8_CASE(4, 2, 2, 2, 2)
+        UHANN_I8_CASE(6, 1, 4, 1, 1)
+        UHANN_I8_CASE(7, 1, 4, 2, 1)
     case 5:
         gram_i8x2_pp_kernel<<<grid, block, 0, stream>>>(q, ldq, m, v, ldv, n, k, qs, vs, out, ldo, tiles_m, tiles_n,
                                                         group_m, sp);
@@ -2205,6 +2219,6 @@ void select_rerank_i8(const SelectArgs& args, hipStream_t stream)
 
 uint32_t gram_i8_k_padding() { return kBKI8; }
 uint32_t gram_i8_max_dim() { return kI8MaxDim; }
-int gram_i8_tile_count() { return 6; }
+int gram_i8_tile_count() { return 8; }
 
 }  // namespace uhann::kern
