@@ -634,7 +634,18 @@ __device__ void rerank_pick(const SelectArgs& a, uint32_t q, const uint32_t* s_c
     const bool vec = a.ld_q % 4U == 0 && a.ld_ref % 4U == 0 && (reinterpret_cast<uintptr_t>(a.queries) & 15U) == 0 &&
                      (reinterpret_cast<uintptr_t>(a.refs) & 15U) == 0;
     constexpr int kWaveRows = ROWS / kSelWaves > 0 ? ROWS / kSelWaves : 1;
-    if (vec) {
+    if (vec && count <= static_cast<uint32_t>(kSelWaves)) {
+        // few candidates (small batches have one block per query and CU):
+        // one per wave, so they stream in parallel
+        const uint32_t wave = static_cast<uint32_t>(t) / 64U;
+        if (wave < count) {
+            const float* rrow[1] = {a.refs + static_cast<uint64_t>(s_cand[wave]) * a.ld_ref};
+            float d[1];
+            wave_distances<1>(qrow, rrow, a.dim, d);
+            if (t % 64 == 0) s_dist[wave] = d[0];
+        }
+        __syncthreads();
+    } else if (vec) {
         const uint32_t wave = static_cast<uint32_t>(t) / 64U;
         for (uint32_t g = wave * kWaveRows; g < count; g += kSelWaves * kWaveRows) {
             const float* rrow[kWaveRows];
