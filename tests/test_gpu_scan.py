@@ -146,14 +146,17 @@ def test_every_tile_bit_exact(uh, oracle_mod, tile, d):
 
 @pytest.mark.parametrize("tile", [(0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0), (0, 4, 8, 0), (0, 8, 8, 0)])
 @pytest.mark.parametrize("metric", ["l2", "l1", "l1w"])
-@pytest.mark.parametrize("d", [222, 300, 32, 1000, 4096, 1, 6, 35])
+@pytest.mark.parametrize("d", [222, 300, 32, 1000, 4096, 1, 6, 35, 256, 257, 1536, 1537])
 def test_pair_tiles_many_reference_blocks(uh, oracle_mod, tile, metric, d):
     # Pair-per-thread scan over several 256-row reference blocks (the last one
     # partial), whole 32-column chunks and tails, empty groups (d = 32, 222),
     # several terms per chain (d = 1000, 4096: the widest rows), chain
     # classes without columns (d = 1, 6, 35), and the L1 weights, for every
     # queries-per-thread width of both bodies (whole 32-chain groups; chain
-    # classes, tr = 8).
+    # classes, tr = 8). d = 256 / 257: the last row with one unit per chain
+    # class and the first with two; d = 1536 / 1537: the last row whose
+    # query columns the chain-class body stages whole and the first it stages
+    # a pass's two groups at a time.
     import torch
 
     r, b = 600, 9
