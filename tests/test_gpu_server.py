@@ -41,6 +41,8 @@ CASES = [
     # about 1750 sampled coordinates: the weighted-L1 pair stage for rows
     # longer than the short pair stages take
     ("flat_l1_mid", "flat", 1, 40, 4000, 800, 0, False, 13),
+    # unweighted L1 rows of at most 1024 coordinates: the short pair stage
+    ("exact_l1_short", "exact", 1, 300, 700, 0, 0, False, 6),
 ]
 
 
