@@ -165,8 +165,10 @@ int uhann_scan_query_device_noalloc(const uhann_scan* scan, const float* device_
                                     void* stream);
 
 /* As uhann_scan_query_device_async with an explicit register tile: rpt
- * chains per thread (1, 2 or 4), tq x tr (query, reference) pairs per block,
- * group_q query tiles per reference sweep; rpt = 0 selects the
+ * chains per thread (1 or 4), tq x tr (query, reference) pairs per block,
+ * group_q query tiles per reference sweep; rpt = 2 selects the
+ * shared-query tile (tq queries and group_q groups of tr reference rows per
+ * block, one chain per thread, the queries staged in LDS); rpt = 0 selects the
  * pair-per-thread scan for rows of at most 4096 coordinates (and fewer than
  * 2^25 reference rows) with tq (1, 2, 4) queries per thread, or with tr = 8
  * its chain-class form with tq (4, 8) queries per thread (the default for

@@ -371,6 +371,167 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
 }
 
 
+// Shared-query tile for small batches: G reference groups of TR rows each
+// and TQ queries per block, one chain per thread (thread t: chain t % 256,
+// group t / 256), so a thread holds TQ x TR partial sums. The block stages
+// each 256-column step's query values in LDS once (double buffered, loaded
+// two steps ahead) and all G groups read them there, so the query
+// traffic from L2 is one TQ x 256 stripe per step for G TR reference rows.
+// Each chain adds its terms in column order and the tree is the reference's
+// (32-lane shuffles, then ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7))), so the bits
+// are those of scan_tile.
+#ifndef UHANN_SQ_DEPTH
+#define UHANN_SQ_DEPTH 4
+#endif
+constexpr int kSqDepth = UHANN_SQ_DEPTH;  // steps of reference values in flight
+static_assert(kSqDepth >= 2 && kSqDepth % 2 == 0, "the query registers alternate between two sets");
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, int TR, int G>
+__device__ __forceinline__ void scan_tile_sq(const ScanArgs& a, uint32_t q0, uint32_t r0, bool stream)
+{
+    constexpr int NT = kChains * G;
+    constexpr int NP = G * TQ * TR;
+    static_assert(TQ % G == 0, "the query stripe must split evenly over the reference groups");
+    constexpr int SQ = TQ / G;  // query values each thread stages per step
+    __shared__ float qs[2][TQ][kChains];
+    __shared__ float wsd[2][kChains];
+    __shared__ float group_sums[kGroups][NP];
+    __shared__ float pair_dist[NP];
+    const int t = threadIdx.x;
+    const uint32_t c = static_cast<uint32_t>(t % kChains);
+    // the group is the same for the whole wave: row addresses in scalar registers
+    const int g = __builtin_amdgcn_readfirstlane(t / kChains);
+    const float* rrow[TR];
+#pragma unroll
+    for (int j = 0; j < TR; ++j) {
+        rrow[j] = a.refs + static_cast<uint64_t>(min(r0 + static_cast<uint32_t>(g * TR + j), a.rows - 1U)) * a.ld_ref;
+    }
+    // this thread stages queries g, g + G, ... at its chain's column
+    const float* qrow[SQ];
+#pragma unroll
+    for (int k = 0; k < SQ; ++k) {
+        qrow[k] = a.queries + static_cast<uint64_t>(min(q0 + static_cast<uint32_t>(g + k * G), a.nq - 1U)) * a.ld_q;
+    }
+    float acc[TQ][TR];
+#pragma unroll
+    for (int i = 0; i < TQ; ++i)
+#pragma unroll
+        for (int j = 0; j < TR; ++j) acc[i][j] = 0.0F;
+
+    const uint32_t steps = (a.dim + kChains - 1) / kChains;
+    // Columns past the row are read at the row's last column: the step that
+    // holds them adds nothing for them, so their values are never used, and
+    // the loads need no branch.
+    const uint32_t last = a.dim - 1U;
+    auto load_refs = [&](uint32_t col, float (&rv)[TR]) __attribute__((always_inline)) {
+        const uint32_t cc = min(col, last);
+#pragma unroll
+        for (int j = 0; j < TR; ++j) rv[j] = stream ? __builtin_nontemporal_load(rrow[j] + cc) : rrow[j][cc];
+    };
+    auto load_queries = [&](uint32_t col, float (&qv)[SQ], float& wv) __attribute__((always_inline)) {
+        const uint32_t cc = min(col, last);
+#pragma unroll
+        for (int k = 0; k < SQ; ++k) qv[k] = qrow[k][cc];
+        wv = WEIGHTED ? a.weights[cc] : 0.0F;
+    };
+    auto store_queries = [&](int buf, const float (&qv)[SQ], float wv) __attribute__((always_inline)) {
+#pragma unroll
+        for (int k = 0; k < SQ; ++k) qs[buf][g + k * G][c] = qv[k];
+        if (WEIGHTED && g == 0) wsd[buf][c] = wv;
+    };
+    // kSqDepth sets of reference registers and two of query registers
+    // rotate with the steps, so nothing is copied between steps: at the end
+    // of step s its sets take the reference values of step s + kSqDepth and
+    // the query values of step s + 3; step s stores the query values of step
+    // s + 1 (loaded two steps earlier) in the other LDS buffer. Rows of no
+    // coordinates add no terms (every distance is zero).
+    if (steps != 0) {
+        float rv[kSqDepth][TR], qa[SQ], qb[SQ], wa, wb;
+        {
+            float q0v[SQ], w0v;
+            load_queries(c, q0v, w0v);
+            store_queries(0, q0v, w0v);
+        }
+        load_queries(c + kChains, qa, wa);
+        load_queries(c + 2 * kChains, qb, wb);
+#pragma unroll
+        for (int u = 0; u < kSqDepth; ++u) load_refs(c + static_cast<uint32_t>(u) * kChains, rv[u]);
+        __syncthreads();
+        auto step = [&](uint32_t s, float (&rs)[TR], float (&qv)[SQ], float& wv) __attribute__((always_inline)) {
+            const uint32_t col = s * kChains + c;
+            const int buf = static_cast<int>(s & 1U);
+            if (col < a.dim) {
+                const float wl = WEIGHTED ? wsd[buf][c] : 0.0F;
+                float ql[TQ];
+#pragma unroll
+                for (int i = 0; i < TQ; ++i) ql[i] = qs[buf][i][c];
+#pragma unroll
+                for (int i = 0; i < TQ; ++i) {
+                    float dd[TR];
+#pragma unroll
+                    for (int j = 0; j < TR; ++j) dd[j] = rs[j] - ql[i];
+                    issue_fence<TR>(dd);
+#pragma unroll
+                    for (int j = 0; j < TR; ++j) acc[i][j] = accumulate_diff<METRIC, WEIGHTED, WFMA>(acc[i][j], dd[j], wl);
+                }
+            }
+            store_queries(buf ^ 1, qv, wv);
+            __syncthreads();
+            load_refs(col + kSqDepth * kChains, rs);
+            load_queries(col + 3 * kChains, qv, wv);
+        };
+        // steps past the row (fewer than kSqDepth) add nothing
+#pragma unroll 1
+        for (uint32_t s = 0; s < steps; s += kSqDepth) {
+#pragma unroll
+            for (int u = 0; u < kSqDepth; u += 2) {
+                step(s + u, rv[u], qa, wa);
+                step(s + u + 1, rv[u + 1], qb, wb);
+            }
+        }
+    }
+    // The reference's tree for each pair: chain offsets 16..1 are lane
+    // offsets inside a 32-lane group; lane 0 of each group stores W.
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) {
+#pragma unroll
+        for (int j = 0; j < TR; ++j) {
+            float v = acc[i][j];
+#pragma unroll
+            for (int off = 16; off >= 1; off /= 2) v = v + __shfl_down(v, off);
+            if (c % 32U == 0) group_sums[c / 32U][(g * TQ + i) * TR + j] = v;
+        }
+    }
+    __syncthreads();
+    for (int p = t; p < NP; p += NT) {
+        const float g0 = group_sums[0][p], g1 = group_sums[1][p];
+        const float g2 = group_sums[2][p], g3 = group_sums[3][p];
+        const float g4 = group_sums[4][p], g5 = group_sums[5][p];
+        const float g6 = group_sums[6][p], g7 = group_sums[7][p];
+        pair_dist[p] = ((g0 + g4) + (g2 + g6)) + ((g1 + g5) + (g3 + g7));
+    }
+    __syncthreads();
+    if (t < TQ) {
+        const uint32_t q = q0 + static_cast<uint32_t>(t);
+        if (q < a.nq) {
+            unsigned long long best = ~0ULL;
+#pragma unroll
+            for (int gg = 0; gg < G; ++gg) {
+#pragma unroll
+                for (int j = 0; j < TR; ++j) {
+                    const uint32_t r = r0 + static_cast<uint32_t>(gg * TR + j);
+                    if (r < a.rows) {
+                        const unsigned long long key =
+                            (static_cast<unsigned long long>(__float_as_uint(pair_dist[(gg * TQ + t) * TR + j])) << 32) |
+                            r;
+                        best = key < best ? key : best;
+                    }
+                }
+            }
+            atomicMin(a.keys + q, best);
+        }
+    }
+}
+
 // Thread-per-query form for rows of at most 128 coordinates (hierarchical p,
 // small flat |I|): thread t owns query q0 + t with its row in registers, the
 // block's kShortRefs references (and the L1 weights) sit in LDS, zero-padded

@@ -200,7 +200,7 @@ void autotune_scan(Scan& scan, Workspace& ws, const float* queries, uint64_t ld_
                               (scan.dim % 4U == 0 || padded);
         std::vector<kern::ScanTile> cands;
         for (const kern::ScanTile& t : kern::scan_tiles()) {
-            if (t.rpt == 1 || aligned4) cands.push_back(t);
+            if (t.rpt != 4 || aligned4) cands.push_back(t);
         }
         if (scan.refs_t.bytes() != 0 && scan.dim != 0 && scan.rows < kern::kPairsMaxRows) {
             for (int tq : {1, 2, 4}) cands.push_back(kern::ScanTile{0, tq, 0, 0});
@@ -311,7 +311,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
     const bool short_ok = a.dim != 0 && a.dim <= kern::scan_short_max_dim();
     const uint64_t short_blocks = ((count + 255) / 256) * ((scan.rows + 127) / 128);
     const Scan::Tuned& tuned = scan.tuned[batch_bucket(count)];
-    if (forced_tile == nullptr && tuned.set && (tuned.tile.rpt <= 1 || aligned4)) {
+    if (forced_tile == nullptr && tuned.set && (tuned.tile.rpt != 4 || aligned4)) {
         tile = tuned.tile;
     } else if (forced_tile == nullptr && short_ok && short_blocks >= device_cus(scan.device)) {
         tile = kern::ScanTile{0, 0, 0, 0};
@@ -343,7 +343,7 @@ void scan_launch(const Scan& scan, const float* queries, uint64_t ld_q, uint64_t
         }
         return;
     }
-    if (tile.rpt > 1 && !aligned4) {
+    if (tile.rpt == 4 && !aligned4) {
         throw std::invalid_argument(
             "vector scan tile needs 16-byte aligned, padded query rows");
     }

@@ -54,7 +54,10 @@ struct ScanArgs {
 // owning `rpt` consecutive residues of the 256-column stride, accumulating a
 // tq x tr tile of (query, reference) pairs. group_q q-tiles share one sweep
 // over r-tiles (L2 locality). weight_fma selects fmaf(w,|d|,acc) for
-// weighted L1 (else rounded product, then add).
+// weighted L1 (else rounded product, then add). rpt 2 is the shared-query
+// tile: group_q reference groups of tr rows and tq queries per block of
+// 256 group_q threads, one chain per thread, the queries' 256-column steps
+// staged in LDS for all groups.
 struct ScanTile {
     int rpt = 1;
     int tq = 8;

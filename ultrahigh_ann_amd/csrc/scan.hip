@@ -69,6 +69,19 @@ scan_direct_kernel(ScanArgs a, uint32_t tiles_q, uint32_t tiles_r, uint32_t grou
     body::scan_tile<METRIC, WEIGHTED, WFMA, RPT, TQ, TR>(a, q0, r0);
 }
 
+// Shared-query tile (body::scan_tile_sq): G reference groups of TR rows and TQ
+// queries per block of 256 G threads; query tiles are the fast index, so the
+// blocks of one reference tile run together (XCD-contiguous ranges).
+template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, int TR, int G>
+__global__ void __launch_bounds__(kChains * G)
+scan_sq_kernel(ScanArgs a, uint32_t tiles_q, uint32_t tiles_r)
+{
+    const uint32_t L = xcd_logical_block(blockIdx.x, tiles_q * tiles_r);
+    const uint32_t q0 = (L % tiles_q) * TQ;
+    const uint32_t r0 = (L / tiles_q) * (TR * G);
+    body::scan_tile_sq<METRIC, WEIGHTED, WFMA, TQ, TR, G>(a, q0, r0, a.stream_refs);
+}
+
 // Pair-per-thread form for short rows (body::scan_pairs): grid.x covers
 // the references 256 at a time, grid.y the queries TQ at a time.
 template <int METRIC, bool WEIGHTED, bool WFMA, int TQ>
@@ -186,6 +199,31 @@ void check_launch(const char* what)
 }
 
 using ScanFn = void (*)(ScanArgs, uint32_t, uint32_t, uint32_t);
+using SqFn = void (*)(ScanArgs, uint32_t, uint32_t);
+
+// Shared-query tiles: ScanTile{2, TQ, TR, G}.
+template <int METRIC, bool WEIGHTED, bool WFMA>
+SqFn pick_sq(int tq, int tr, int g)
+{
+#define UHANN_SQ(Q, T, GG)                                                        \
+    if (tq == Q && tr == T && g == GG) return scan_sq_kernel<METRIC, WEIGHTED, WFMA, Q, T, GG>;
+    UHANN_SQ(16, 2, 4)
+    UHANN_SQ(16, 4, 2)
+    UHANN_SQ(16, 4, 4)
+    UHANN_SQ(8, 4, 2)
+    UHANN_SQ(4, 4, 4)
+#undef UHANN_SQ
+    return nullptr;
+}
+
+SqFn select_sq(Metric metric, bool weighted, bool wfma, const ScanTile& t)
+{
+    if (t.rpt != 2) return nullptr;
+    if (metric == Metric::l2) return pick_sq<0, false, false>(t.tq, t.tr, t.group_q);
+    if (!weighted) return pick_sq<1, false, false>(t.tq, t.tr, t.group_q);
+    if (wfma) return pick_sq<1, true, true>(t.tq, t.tr, t.group_q);
+    return pick_sq<1, true, false>(t.tq, t.tr, t.group_q);
+}
 
 template <int METRIC, bool WEIGHTED, bool WFMA>
 ScanFn pick(int rpt, int tq, int tr)
@@ -234,13 +272,15 @@ const std::vector<ScanTile>& scan_tiles()
         {1, 8, 8, 8},  {1, 12, 12, 8}, {1, 16, 8, 4}, {1, 8, 16, 2}, {1, 4, 16, 1}, {1, 2, 16, 1},
         {1, 1, 16, 1}, {4, 1, 8, 1},   {4, 2, 8, 1},  {4, 4, 8, 1},  {4, 8, 4, 2},  {1, 1, 4, 1},
         {1, 2, 8, 1},  {1, 4, 8, 1},   {1, 16, 4, 4}, {4, 1, 4, 1},  {4, 2, 4, 1},  {4, 4, 4, 1},
-        {4, 1, 2, 1},  {4, 1, 1, 1},   {4, 2, 2, 1},  {4, 4, 2, 1},
+        {4, 1, 2, 1},  {4, 1, 1, 1},   {4, 2, 2, 1},  {4, 4, 2, 1},   {2, 16, 2, 4}, {2, 16, 4, 2},
+        {2, 16, 4, 4}, {2, 8, 4, 2},   {2, 4, 4, 4},
     };
     return tiles;
 }
 
 bool scan_tile_supported(const ScanTile& tile)
 {
+    if (tile.rpt == 2) return select_sq(Metric::l2, false, false, tile) != nullptr;
     return select_scan(Metric::l2, false, false, tile) != nullptr && tile.group_q >= 1;
 }
 
@@ -262,6 +302,20 @@ void finalize_keys(const unsigned long long* keys, uint32_t n, uint64_t* out_idx
 void scan_direct(Metric metric, bool weight_fma, const ScanArgs& args, const ScanTile& tile,
                  hipStream_t stream)
 {
+    if (tile.rpt == 2) {
+        const SqFn sq = select_sq(metric, args.weights != nullptr, weight_fma, tile);
+        if (sq == nullptr) throw std::invalid_argument("unsupported scan tile");
+        if (args.nq == 0 || args.rows == 0) return;
+        const uint32_t tiles_q = (args.nq + tile.tq - 1) / tile.tq;
+        const uint32_t per_block = static_cast<uint32_t>(tile.tr * tile.group_q);
+        const uint32_t tiles_r = (args.rows + per_block - 1) / per_block;
+        const uint64_t blocks = static_cast<uint64_t>(tiles_q) * tiles_r;
+        if (blocks > 0x7fffffffULL) throw std::length_error("scan grid exceeds the launch limit");
+        hipLaunchKernelGGL(sq, dim3(static_cast<uint32_t>(blocks)), dim3(kChains * tile.group_q), 0, stream, args,
+                           tiles_q, tiles_r);
+        check_launch("shared-query scan kernel");
+        return;
+    }
     const ScanFn fn = select_scan(metric, args.weights != nullptr, weight_fma, tile);
     if (fn == nullptr) throw std::invalid_argument("unsupported scan tile");
     if (args.nq == 0 || args.rows == 0) return;
