@@ -15,7 +15,7 @@ TILES = [(1, 8, 8, 8), (1, 12, 12, 8), (1, 16, 8, 4), (1, 8, 16, 2), (1, 4, 16, 
          (1, 1, 16, 1), (4, 1, 8, 1), (4, 2, 8, 1), (4, 4, 8, 1), (4, 8, 4, 2),
          (1, 1, 4, 1), (1, 2, 8, 1), (1, 4, 8, 1), (1, 16, 4, 4), (4, 1, 4, 1), (4, 2, 4, 1), (4, 4, 4, 1),
          (4, 1, 2, 1), (4, 1, 1, 1), (4, 2, 2, 1), (4, 4, 2, 1),
-         (2, 16, 2, 4), (2, 16, 4, 2), (2, 16, 4, 4), (2, 8, 4, 2), (2, 4, 4, 4),
+         (2, 16, 4, 4), (2, 8, 4, 4), (2, 8, 4, 2),
          (0, 1, 0, 0), (0, 2, 0, 0), (0, 4, 0, 0), (0, 4, 8, 0), (0, 8, 8, 0), (0, 0, 0, 0)]
 
 

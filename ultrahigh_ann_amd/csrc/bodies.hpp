@@ -380,11 +380,7 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
 // Each chain adds its terms in column order and the tree is the reference's
 // (32-lane shuffles, then ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7))), so the bits
 // are those of scan_tile.
-#ifndef UHANN_SQ_DEPTH
-#define UHANN_SQ_DEPTH 4
-#endif
-constexpr int kSqDepth = UHANN_SQ_DEPTH;  // steps of reference values in flight
-static_assert(kSqDepth >= 2 && kSqDepth % 2 == 0, "the query registers alternate between two sets");
+constexpr int kSqDepth = 4;  // steps of reference values in flight (the step loop's unroll)
 template <int METRIC, bool WEIGHTED, bool WFMA, int TQ, int TR, int G>
 __device__ __forceinline__ void scan_tile_sq(const ScanArgs& a, uint32_t q0, uint32_t r0, bool stream)
 {
@@ -392,8 +388,13 @@ __device__ __forceinline__ void scan_tile_sq(const ScanArgs& a, uint32_t q0, uin
     constexpr int NP = G * TQ * TR;
     static_assert(TQ % G == 0, "the query stripe must split evenly over the reference groups");
     constexpr int SQ = TQ / G;  // query values each thread stages per step
-    __shared__ float qs[2][TQ][kChains];
-    __shared__ float wsd[2][kChains];
+    // Without weights two steps share an LDS buffer (one barrier per two
+    // steps); the weighted form keeps one step per buffer, as its extra
+    // registers for two steps would spill.
+    constexpr bool kPairs = !WEIGHTED;
+    constexpr int kSlots = kPairs ? 4 : 2;
+    __shared__ float qs[kSlots][TQ][kChains];
+    __shared__ float wsd[kSlots][kChains];
     __shared__ float group_sums[kGroups][NP];
     __shared__ float pair_dist[NP];
     const int t = threadIdx.x;
@@ -440,52 +441,107 @@ __device__ __forceinline__ void scan_tile_sq(const ScanArgs& a, uint32_t q0, uin
     };
     // kSqDepth sets of reference registers and two of query registers
     // rotate with the steps, so nothing is copied between steps: at the end
-    // of step s its sets take the reference values of step s + kSqDepth and
-    // the query values of step s + 3; step s stores the query values of step
-    // s + 1 (loaded two steps earlier) in the other LDS buffer. Rows of no
-    // coordinates add no terms (every distance is zero).
+    // of step s its reference set takes the values of step s + kSqDepth; the
+    // query values reach LDS one buffer (one or two steps) ahead of use and
+    // are loaded one buffer before that. Rows of no coordinates add no terms
+    // (every distance is zero).
     if (steps != 0) {
-        float rv[kSqDepth][TR], qa[SQ], qb[SQ], wa, wb;
-        {
-            float q0v[SQ], w0v;
-            load_queries(c, q0v, w0v);
-            store_queries(0, q0v, w0v);
-        }
-        load_queries(c + kChains, qa, wa);
-        load_queries(c + 2 * kChains, qb, wb);
-#pragma unroll
-        for (int u = 0; u < kSqDepth; ++u) load_refs(c + static_cast<uint32_t>(u) * kChains, rv[u]);
-        __syncthreads();
-        auto step = [&](uint32_t s, float (&rs)[TR], float (&qv)[SQ], float& wv) __attribute__((always_inline)) {
-            const uint32_t col = s * kChains + c;
-            const int buf = static_cast<int>(s & 1U);
-            if (col < a.dim) {
-                const float wl = WEIGHTED ? wsd[buf][c] : 0.0F;
-                float ql[TQ];
-#pragma unroll
-                for (int i = 0; i < TQ; ++i) ql[i] = qs[buf][i][c];
-#pragma unroll
-                for (int i = 0; i < TQ; ++i) {
-                    float dd[TR];
-#pragma unroll
-                    for (int j = 0; j < TR; ++j) dd[j] = rs[j] - ql[i];
-                    issue_fence<TR>(dd);
-#pragma unroll
-                    for (int j = 0; j < TR; ++j) acc[i][j] = accumulate_diff<METRIC, WEIGHTED, WFMA>(acc[i][j], dd[j], wl);
-                }
+        if constexpr (kPairs) {
+            // pair P = steps 2P, 2P + 1: its query values take LDS slots 2 (P & 1)
+            // and 2 (P & 1) + 1, and one barrier serves both steps
+            float rv[kSqDepth][TR], qa[2][SQ], qb[2][SQ], wa[2], wb[2];
+            auto load_pair = [&](uint32_t col, float (&qv)[2][SQ], float (&wv)[2]) __attribute__((always_inline)) {
+                load_queries(col, qv[0], wv[0]);
+                load_queries(col + kChains, qv[1], wv[1]);
+            };
+            {
+                float q0v[2][SQ], w0v[2];
+                load_pair(c, q0v, w0v);
+                store_queries(0, q0v[0], w0v[0]);
+                store_queries(1, q0v[1], w0v[1]);
             }
-            store_queries(buf ^ 1, qv, wv);
-            __syncthreads();
-            load_refs(col + kSqDepth * kChains, rs);
-            load_queries(col + 3 * kChains, qv, wv);
-        };
-        // steps past the row (fewer than kSqDepth) add nothing
-#pragma unroll 1
-        for (uint32_t s = 0; s < steps; s += kSqDepth) {
+            load_pair(c + 2 * kChains, qa, wa);
+            load_pair(c + 4 * kChains, qb, wb);
 #pragma unroll
-            for (int u = 0; u < kSqDepth; u += 2) {
-                step(s + u, rv[u], qa, wa);
-                step(s + u + 1, rv[u + 1], qb, wb);
+            for (int u = 0; u < kSqDepth; ++u) load_refs(c + static_cast<uint32_t>(u) * kChains, rv[u]);
+            __syncthreads();
+            auto step = [&](uint32_t s, int slot, float (&rs)[TR]) __attribute__((always_inline)) {
+                const uint32_t col = s * kChains + c;
+                if (col < a.dim) {
+                    const float wl = WEIGHTED ? wsd[slot][c] : 0.0F;
+                    float ql[TQ];
+#pragma unroll
+                    for (int i = 0; i < TQ; ++i) ql[i] = qs[slot][i][c];
+#pragma unroll
+                    for (int i = 0; i < TQ; ++i) {
+                        float dd[TR];
+#pragma unroll
+                        for (int j = 0; j < TR; ++j) dd[j] = rs[j] - ql[i];
+                        issue_fence<TR>(dd);
+#pragma unroll
+                        for (int j = 0; j < TR; ++j) acc[i][j] = accumulate_diff<METRIC, WEIGHTED, WFMA>(acc[i][j], dd[j], wl);
+                    }
+                }
+                load_refs(col + kSqDepth * kChains, rs);
+            };
+            auto pair_end = [&](uint32_t s, int buf, float (&qv)[2][SQ], float (&wv)[2]) __attribute__((always_inline)) {
+                store_queries(2 * (buf ^ 1), qv[0], wv[0]);
+                store_queries(2 * (buf ^ 1) + 1, qv[1], wv[1]);
+                __syncthreads();
+                load_pair(s * kChains + c + 6 * kChains, qv, wv);
+            };
+#pragma unroll 1
+            for (uint32_t s = 0; s < steps; s += 4) {
+                step(s, 0, rv[0]);
+                step(s + 1, 1, rv[1]);
+                pair_end(s, 0, qa, wa);
+                step(s + 2, 2, rv[2]);
+                step(s + 3, 3, rv[3]);
+                pair_end(s + 2, 1, qb, wb);
+            }
+        } else {
+            float rv[kSqDepth][TR], qa[SQ], qb[SQ], wa, wb;
+            {
+                float q0v[SQ], w0v;
+                load_queries(c, q0v, w0v);
+                store_queries(0, q0v, w0v);
+            }
+            load_queries(c + kChains, qa, wa);
+            load_queries(c + 2 * kChains, qb, wb);
+#pragma unroll
+            for (int u = 0; u < kSqDepth; ++u) load_refs(c + static_cast<uint32_t>(u) * kChains, rv[u]);
+            __syncthreads();
+            auto step = [&](uint32_t s, float (&rs)[TR], float (&qv)[SQ], float& wv) __attribute__((always_inline)) {
+                const uint32_t col = s * kChains + c;
+                const int buf = static_cast<int>(s & 1U);
+                if (col < a.dim) {
+                    const float wl = WEIGHTED ? wsd[buf][c] : 0.0F;
+                    float ql[TQ];
+#pragma unroll
+                    for (int i = 0; i < TQ; ++i) ql[i] = qs[buf][i][c];
+#pragma unroll
+                    for (int i = 0; i < TQ; ++i) {
+                        float dd[TR];
+#pragma unroll
+                        for (int j = 0; j < TR; ++j) dd[j] = rs[j] - ql[i];
+                        issue_fence<TR>(dd);
+#pragma unroll
+                        for (int j = 0; j < TR; ++j) acc[i][j] = accumulate_diff<METRIC, WEIGHTED, WFMA>(acc[i][j], dd[j], wl);
+                    }
+                }
+                store_queries(buf ^ 1, qv, wv);
+                __syncthreads();
+                load_refs(col + kSqDepth * kChains, rs);
+                load_queries(col + 3 * kChains, qv, wv);
+            };
+            // steps past the row (fewer than kSqDepth) add nothing
+#pragma unroll 1
+            for (uint32_t s = 0; s < steps; s += kSqDepth) {
+#pragma unroll
+                for (int u = 0; u < kSqDepth; u += 2) {
+                    step(s + u, rv[u], qa, wa);
+                    step(s + u + 1, rv[u + 1], qb, wb);
+                }
             }
         }
     }

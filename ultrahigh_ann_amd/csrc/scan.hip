@@ -207,11 +207,9 @@ SqFn pick_sq(int tq, int tr, int g)
 {
 #define UHANN_SQ(Q, T, GG)                                                        \
     if (tq == Q && tr == T && g == GG) return scan_sq_kernel<METRIC, WEIGHTED, WFMA, Q, T, GG>;
-    UHANN_SQ(16, 2, 4)
-    UHANN_SQ(16, 4, 2)
     UHANN_SQ(16, 4, 4)
+    UHANN_SQ(8, 4, 4)
     UHANN_SQ(8, 4, 2)
-    UHANN_SQ(4, 4, 4)
 #undef UHANN_SQ
     return nullptr;
 }
@@ -272,8 +270,8 @@ const std::vector<ScanTile>& scan_tiles()
         {1, 8, 8, 8},  {1, 12, 12, 8}, {1, 16, 8, 4}, {1, 8, 16, 2}, {1, 4, 16, 1}, {1, 2, 16, 1},
         {1, 1, 16, 1}, {4, 1, 8, 1},   {4, 2, 8, 1},  {4, 4, 8, 1},  {4, 8, 4, 2},  {1, 1, 4, 1},
         {1, 2, 8, 1},  {1, 4, 8, 1},   {1, 16, 4, 4}, {4, 1, 4, 1},  {4, 2, 4, 1},  {4, 4, 4, 1},
-        {4, 1, 2, 1},  {4, 1, 1, 1},   {4, 2, 2, 1},  {4, 4, 2, 1},   {2, 16, 2, 4}, {2, 16, 4, 2},
-        {2, 16, 4, 4}, {2, 8, 4, 2},   {2, 4, 4, 4},
+        {4, 1, 2, 1},  {4, 1, 1, 1},   {4, 2, 2, 1},  {4, 4, 2, 1},   {2, 16, 4, 4}, {2, 8, 4, 4},
+        {2, 8, 4, 2},
     };
     return tiles;
 }
