@@ -486,7 +486,7 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
         ws = index.make_query_workspace(64)
         table = index.autotune(queries, 16, ws)
         rows = {"autotuned": {f"<= {t['max_batch']}": [t["rpt"], t["tq"], t["tr"], t["group_q"]] for t in table}}
-        for b in (1, 4, 16):
+        for b in (1, 4, 8, 16):
             wall, ms = timed(lambda: index.query_device_async(queries, b, ws, oi, od), 5, 2, torch)
             t = float(np.median(ms)) * 1e-3
             bytes_alg = 4.0 * r * d + 4.0 * b * d + 12.0 * b
@@ -508,7 +508,7 @@ def suite(uh, torch, refs, queries, refs_np, exact_idx_l2, args):
         # references' int8 planes (half the binary32 bytes) and returns the
         # binary32 path's rows and bits.
         index.enable_mixed(uh.STRATEGY_GRAM_INT8)
-        for b in (1, 4, 16):
+        for b in (1, 4, 8, 16):
             index.query_device_async(queries, b, ws, oi, od)
             want = (oi[:b].cpu().numpy().copy(), od[:b].cpu().numpy().view(np.uint32).copy())
             wall, ms = timed(lambda: index.query_device_async(queries, b, ws, oi, od, strategy=uh.STRATEGY_GRAM_INT8),
