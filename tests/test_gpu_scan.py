@@ -213,7 +213,7 @@ def test_short_rows_bit_exact(uh, oracle_mod, d, r, b):
         np.testing.assert_array_equal(dist.view(np.uint32), exp_dist.view(np.uint32))
 
 
-STREAM_TILES = [t for t in TILES if t[0] == 4]
+STREAM_TILES = [t for t in TILES if t[0] in (4, 2)]
 
 
 @pytest.mark.parametrize("tile", STREAM_TILES)
@@ -239,6 +239,36 @@ def test_stream_tiles_bit_exact(uh, oracle_mod, tile, d):
             torch.cuda.synchronize()
             np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
             np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+
+
+@pytest.mark.parametrize("tile", [t for t in TILES if t[0] == 2])
+@pytest.mark.parametrize("metric", ["l1", "l1w"])
+@pytest.mark.parametrize("d", [1000, 257, 33, 1])
+def test_shared_query_tiles_l1(uh, oracle_mod, tile, metric, d):
+    """Shared-query tiles (one chain per thread, the block's queries staged in
+    LDS) on L1 and weighted L1 indexes with several query tiles, the last one
+    partial, and several reference blocks, the last one partial: the
+    weighted form stages one step per LDS buffer, the others two."""
+    import torch
+
+    r, b = 45, 2 * tile[1] + 3
+    refs, q = _data(17 + d + tile[1], r, d, b)
+    if metric == "l1":
+        index = uh.HipDenseL1ScanIndex(refs)
+        expect = oracle_mod.k5_distances(refs, q)
+    else:
+        w = (rng(d + 5).random(d, dtype=np.float32) * 3 + 0.1).astype(F32)
+        index = uh.HipDenseL1ScanIndex(refs, w)
+        expect = oracle_mod.k5_distances(refs, q, w, 0)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(expect)
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    index.query_device_tile(dq, b, ws, oi, od, tile, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
 
 
 def test_device_batch_and_async(uh, oracle_mod):
