@@ -412,6 +412,39 @@ def test_pair_scan_batches_beyond_grid_limit(uh, oracle_mod, tq):
     np.testing.assert_array_equal(od.cpu().numpy()[pick].view(np.uint32), exp_dist.view(np.uint32))
 
 
+@pytest.mark.parametrize("metric", ["l2", "l1w"])
+def test_short_row_scan_batches_beyond_grid_limit(uh, oracle_mod, metric):
+    """More query blocks than one grid dimension holds (65535 x 256 queries):
+    the thread-per-query scan runs the batch in slices. Queries around the
+    slice boundary and a spread sample agree with the oracle bit for bit."""
+    import torch
+
+    r, d = 9, 3
+    b = 65535 * 256 + 300
+    g = rng(91)
+    refs = (g.random((r, d), dtype=np.float32) * 4).astype(F32)
+    q = (g.random((b, d), dtype=np.float32) * 4).astype(F32)
+    if metric == "l2":
+        index = uh.HipDenseL2ScanIndex(refs)
+    else:
+        w = (g.random(d, dtype=np.float32) * 3 + 0.1).astype(F32)
+        index = uh.HipDenseL1ScanIndex(refs, w)
+    ws = index.make_query_workspace(b)
+    dq = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    index.query_device_tile(dq, b, ws, oi, od, (0, 0, 0, 0), stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    edge = 65535 * 256
+    pick = np.unique(np.concatenate([np.arange(edge - 3, edge + 3), np.arange(0, b, 104729), np.arange(b - 3, b)]))
+    sub = q[pick]
+    expect = oracle_mod.k2_distances(refs, sub) if metric == "l2" else oracle_mod.k5_distances(refs, sub, w, 0)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(expect)
+    np.testing.assert_array_equal(oi.cpu().numpy()[pick].astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy()[pick].view(np.uint32), exp_dist.view(np.uint32))
+    del dq, oi, od, ws
+
+
 def test_autotune_large_batch_short_rows(uh, oracle_mod):
     """Autotuning up to 65536 queries on short rows tries the pair tiles at
     every bucket (the tq = 1 tile needs 65536 query tiles there) and keeps

@@ -352,30 +352,38 @@ void scan_short(Metric metric, bool weight_fma, const float* refs, uint64_t ld_r
 {
     if (nq == 0 || rows == 0) return;
     if (dim == 0 || dim > scan_short_max_dim()) throw std::invalid_argument("short-row scan supports 1 <= dim <= 128");
-    const dim3 grid((rows + body::kShortRefs - 1) / body::kShortRefs, (nq + body::kShortThreads - 1) / body::kShortThreads);
-    if (grid.y > 65535U) throw std::length_error("short-row scan batch exceeds the grid limit");
     const bool weighted = weights != nullptr;
     const int m = metric == Metric::l2 ? 0 : 1;
+    // grid.y carries the query blocks; batches beyond 65535 of them go in
+    // slices (a 64-bit counter, so the last step cannot wrap past 2^32)
+    const uint64_t slice = 65535ULL * static_cast<uint64_t>(body::kShortThreads);
+    for (uint64_t q0 = 0; q0 < nq; q0 += slice) {
+        const uint32_t n = static_cast<uint32_t>(nq - q0 < slice ? nq - q0 : slice);
+        const float* qs = queries + q0 * ld_q;
+        unsigned long long* ks = keys + q0;
+        const dim3 grid((rows + body::kShortRefs - 1) / body::kShortRefs,
+                        (n + body::kShortThreads - 1) / body::kShortThreads);
 #define UHANN_SHORT(M, W, F, D)                                                                                 \
     hipLaunchKernelGGL((scan_short_kernel<M, W, F, D>), grid, dim3(body::kShortThreads), 0, stream, refs, ld_ref, \
-                       rows, queries, ld_q, nq, weights, dim, keys)
+                       rows, qs, ld_q, n, weights, dim, ks)
 #define UHANN_SHORT_D(D)                                 \
     if (m == 0) UHANN_SHORT(0, false, false, D);         \
     else if (!weighted) UHANN_SHORT(1, false, false, D); \
     else if (weight_fma) UHANN_SHORT(1, true, true, D);  \
     else UHANN_SHORT(1, true, false, D);
-    if (dim <= 16) {
-        UHANN_SHORT_D(16)
-    } else if (dim <= 32) {
-        UHANN_SHORT_D(32)
-    } else if (dim <= 64) {
-        UHANN_SHORT_D(64)
-    } else {
-        UHANN_SHORT_D(128)
+        if (dim <= 16) {
+            UHANN_SHORT_D(16)
+        } else if (dim <= 32) {
+            UHANN_SHORT_D(32)
+        } else if (dim <= 64) {
+            UHANN_SHORT_D(64)
+        } else {
+            UHANN_SHORT_D(128)
+        }
+        check_launch("short-row scan kernel");
     }
 #undef UHANN_SHORT_D
 #undef UHANN_SHORT
-    check_launch("short-row scan kernel");
 }
 
 uint32_t scan_pairs_max_dim() { return 4096; }
@@ -399,10 +407,11 @@ void scan_pairs(Metric metric, bool weight_fma, const float* refs_t, uint64_t ld
     }
     const bool cls = classes;
     // grid.y carries the query tile; batches beyond 65535 tiles go in slices.
-    const uint32_t slice = 65535U * static_cast<uint32_t>(tq);
-    for (uint32_t q0 = 0; q0 < nq; q0 += slice) {
-        const uint32_t n = nq - q0 < slice ? nq - q0 : slice;
-        const float* qs = queries + static_cast<uint64_t>(q0) * ld_q;
+    // (a 64-bit counter, so the last step cannot wrap past 2^32)
+    const uint64_t slice = 65535ULL * static_cast<uint64_t>(tq);
+    for (uint64_t q0 = 0; q0 < nq; q0 += slice) {
+        const uint32_t n = static_cast<uint32_t>(nq - q0 < slice ? nq - q0 : slice);
+        const float* qs = queries + q0 * ld_q;
         unsigned long long* ks = keys + q0;
         const dim3 grid((rows + 255) / 256, (n + tq - 1) / tq);
 #define UHANN_PAIRS(M, W, F, Q)                                                                       \
