@@ -412,6 +412,30 @@ def test_pair_scan_batches_beyond_grid_limit(uh, oracle_mod, tq):
     np.testing.assert_array_equal(od.cpu().numpy()[pick].view(np.uint32), exp_dist.view(np.uint32))
 
 
+def test_distance_table_batches_beyond_grid_limit(uh, oracle_mod):
+    """More query tiles than one grid dimension holds (65535 x 64 queries):
+    the binary64 table runs the batch in slices; rows around the slice
+    boundary and a spread sample equal the oracle's table bit for bit."""
+    import torch
+
+    r, d = 3, 2
+    b = 65535 * 64 + 70
+    g = np.random.default_rng(5)
+    refs = (g.random((r, d), dtype=np.float32) * 3).astype(np.float32)
+    q = (g.random((b, d), dtype=np.float32) * 3).astype(np.float32)
+    index = uh.HipExactL2Index(refs)
+    dq = torch.from_numpy(q).cuda()
+    out = torch.full((b, r), float("nan"), dtype=torch.float64, device="cuda")
+    index.distance_table(dq, b, out)
+    torch.cuda.synchronize()
+    edge = 65535 * 64
+    pick = np.unique(np.concatenate([np.arange(edge - 3, edge + 3), np.arange(0, b, 99991), np.arange(b - 3, b)]))
+    got = out.cpu().numpy()[pick]
+    want = np.sqrt(oracle_mod.fp64_table(False, refs, q[pick]))
+    np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    del dq, out
+
+
 @pytest.mark.parametrize("metric", ["l2", "l1w"])
 def test_short_row_scan_batches_beyond_grid_limit(uh, oracle_mod, metric):
     """More query blocks than one grid dimension holds (65535 x 256 queries):

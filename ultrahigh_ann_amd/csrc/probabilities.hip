@@ -244,14 +244,21 @@ void distance_table(bool l1, const float* queries, uint64_t ldq, uint32_t nq, co
                     uint32_t nr, uint32_t dim, double* out, uint64_t ldo, hipStream_t stream)
 {
     if (nq == 0 || nr == 0) return;
-    const dim3 grid((nr + kPairTile - 1) / kPairTile, (nq + kPairTile - 1) / kPairTile);
-    if (grid.y > 65535U) throw std::length_error("distance table grid exceeds the launch limit");
-    if (l1) {
-        distance_table_kernel<true><<<grid, kPairThreads, 0, stream>>>(queries, ldq, nq, refs, ldr, nr, dim, out, ldo);
-    } else {
-        distance_table_kernel<false><<<grid, kPairThreads, 0, stream>>>(queries, ldq, nq, refs, ldr, nr, dim, out, ldo);
+    // grid.y carries the query tiles; batches beyond 65535 of them go in
+    // slices (a 64-bit counter, so the last step cannot wrap past 2^32)
+    const uint64_t slice = 65535ULL * kPairTile;
+    for (uint64_t q0 = 0; q0 < nq; q0 += slice) {
+        const uint32_t n = static_cast<uint32_t>(nq - q0 < slice ? nq - q0 : slice);
+        const float* qs = queries + q0 * ldq;
+        double* os = out + q0 * ldo;
+        const dim3 grid((nr + kPairTile - 1) / kPairTile, (n + kPairTile - 1) / kPairTile);
+        if (l1) {
+            distance_table_kernel<true><<<grid, kPairThreads, 0, stream>>>(qs, ldq, n, refs, ldr, nr, dim, os, ldo);
+        } else {
+            distance_table_kernel<false><<<grid, kPairThreads, 0, stream>>>(qs, ldq, n, refs, ldr, nr, dim, os, ldo);
+        }
+        check_launch("distance table kernel");
     }
-    check_launch("distance table kernel");
 }
 
 void importance_probabilities(bool l1, const float* refs, uint64_t ld, uint32_t rows, uint32_t dim,
