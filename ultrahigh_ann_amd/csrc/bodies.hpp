@@ -374,9 +374,10 @@ __device__ __forceinline__ void scan_tile(const ScanArgs& a, uint32_t q0, uint32
 // Shared-query tile for small batches: G reference groups of TR rows each
 // and TQ queries per block, one chain per thread (thread t: chain t % 256,
 // group t / 256), so a thread holds TQ x TR partial sums. The block stages
-// each 256-column step's query values in LDS once (double buffered, loaded
-// two steps ahead) and all G groups read them there, so the query
-// traffic from L2 is one TQ x 256 stripe per step for G TR reference rows.
+// each 256-column step's query values in LDS once (two buffers of one or two
+// steps each, filled one buffer ahead of use) and all G groups read them
+// there, so the query traffic from L2 is one TQ x 256 stripe per step for
+// G TR reference rows.
 // Each chain adds its terms in column order and the tree is the reference's
 // (32-lane shuffles, then ((W0+W4)+(W2+W6))+((W1+W5)+(W3+W7))), so the bits
 // are those of scan_tile.
