@@ -271,6 +271,28 @@ def test_shared_query_tiles_l1(uh, oracle_mod, tile, metric, d):
     np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
 
 
+@pytest.mark.parametrize("tile", [t for t in TILES if t[0] == 2])
+def test_shared_query_tiles_strided_queries(uh, oracle_mod, tile):
+    """Shared-query tiles on device query rows with a stride that is not a
+    multiple of four floats (ld = d + 13): same bits as the oracle."""
+    import torch
+
+    r, d = 45, 1000
+    b = 2 * tile[1] + 3
+    refs, q = _data(23 + tile[1] + tile[3], r, d, b)
+    index = uh.HipDenseL2ScanIndex(refs)
+    ws = index.make_query_workspace(b)
+    exp_idx, exp_dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q))
+    padded = torch.full((b, d + 13), float("nan"), dtype=torch.float32, device="cuda")
+    padded[:, :d] = torch.from_numpy(q).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    index.query_device_tile(padded, b, ws, oi, od, tile, stream=torch.cuda.current_stream(), ld_queries=d + 13)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), exp_idx)
+    np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), exp_dist.view(np.uint32))
+
+
 def test_device_batch_and_async(uh, oracle_mod):
     import torch
 
