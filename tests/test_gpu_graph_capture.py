@@ -68,3 +68,45 @@ def test_query_graph_replay(uh, strategy, escalate):
         # eager work on the same workspace between two replays
         o_idx, _ = index.query_batch(qb if which == 0 else qa, b, ws, strat)
         np.testing.assert_array_equal(o_idx, eager[1 - which][0])
+
+
+def test_shared_query_tile_graph_replay(uh, oracle_mod):
+    """The shared-query scan tile (16 queries, four groups of four rows per
+    block) captured into a HIP graph: every replay, with eager calls on the
+    same workspace between them, returns the oracle's rows and distances."""
+    import torch
+
+    r, d, b = 300, 1000, 16
+    tile = (2, 16, 4, 4)
+    refs, qa, qb = _batches(57, r, d, b, False)
+    index = uh.HipDenseL2ScanIndex(refs)
+    ws = index.make_query_workspace(b)
+    expect = {}
+    for which, q in ((0, qa), (1, qb)):
+        idx, dist = oracle_mod.fp32_argmin(oracle_mod.k2_distances(refs, q))
+        expect[which] = (idx, dist)
+    dq = torch.from_numpy(qa).cuda()
+    oi = torch.zeros(b, dtype=torch.int64, device="cuda")
+    od = torch.zeros(b, dtype=torch.float32, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        index.query_device_tile(dq, b, ws, oi, od, tile, stream=side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        index.query_device_tile(dq, b, ws, oi, od, tile, stream=torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    hq = {0: torch.from_numpy(qa).cuda(), 1: torch.from_numpy(qb).cuda()}
+    for rep, which in enumerate((0, 1, 1, 0, 1)):
+        dq.copy_(hq[which])
+        oi.fill_(-1)
+        od.fill_(-1.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        e_idx, e_dist = expect[which]
+        np.testing.assert_array_equal(oi.cpu().numpy().astype(np.uint64), e_idx, err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(od.cpu().numpy().view(np.uint32), e_dist.view(np.uint32), err_msg=f"replay {rep}")
+        o_idx, _ = index.query_batch(qb if which == 0 else qa, b, ws)
+        np.testing.assert_array_equal(o_idx, expect[1 - which][0])
